@@ -1,0 +1,184 @@
+/*
+ * swehr.h - C ABI of the MI355X-native SWEHR coupled time step.
+ *
+ * The reference (SWEHR.c) has no library seam: the time step is written inline in
+ * main() (SWEHR.c:808-1699) and every parameter either comes from the five input
+ * files (SWEHR.c:468-522) or is hard-coded (SWEHR.c:236-246, 617-650).  This header
+ * is the seam: a host driver keeps main()'s control flow (parse, initial condition,
+ * outputs) and calls these entry points for the loop body (boundary fill, flux and
+ * source pass, CFL dt, update pass; SWEHR.c:810-1591) plus the outlet/probe sampling
+ * (SWEHR.c:1593-1676) that the loop does after each step.
+ *
+ * Conventions
+ *  - Plain C types only.  Host arrays are borrowed for the duration of a call.
+ *  - Rasters are "i-major": value (i,j), i=1..nx across, j=1..ny downslope, sits at
+ *    [(i-1)*ny + (j-1)], the order topoin.txt is read in (SWEHR.c:223-227).
+ *  - Every entry point returns an int status (enum swehr_status); the reference's
+ *    exit(1)/print-and-stop paths (SWEHR.c:29-36, 1413) become codes.
+ *  - One host thread per handle; different handles may be used from different threads.
+ */
+#ifndef SWEHR_H
+#define SWEHR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SWEHR_MAX_CLASSES 16
+#define SWEHR_STAGE_COLS 24   /* STAGE[1..24] columns, SWEHR.c:600, 1608-1651 */
+#define SWEHR_FLOW_COLS 18    /* SAVEFLOW[1..18] columns, SWEHR.c:601, 1653-1675 */
+#define SWEHR_STAGE_MAX 30000 /* records the reference can hold, SWEHR.c:600-601 */
+
+enum swehr_status {
+    SWEHR_OK = 0,
+    SWEHR_ERR_CFL = 1,       /* courant > 1 (SWEHR.c:1413): the step is taken, then the run stops */
+    SWEHR_ERR_NONFINITE = 2, /* non-finite value where a finite one is required */
+    SWEHR_ERR_OOM = 3,       /* device or host allocation failed (SWEHR.c:116-169 exit(1)) */
+    SWEHR_ERR_BAD_INPUT = 4, /* invalid sizes or parameters */
+    SWEHR_ERR_HIP = 5,       /* HIP runtime error */
+    SWEHR_ERR_STAGE_FULL = 6,/* more than SWEHR_STAGE_MAX outlet records */
+    SWEHR_ERR_NO_DEVICE = 7, /* no usable GPU */
+    SWEHR_ERR_BUSY = 8       /* capacity exhausted (serving admission) */
+};
+
+enum swehr_precision {
+    SWEHR_FP64 = 0,          /* plain: all state and arithmetic in FP64, like the reference */
+    SWEHR_FP32 = 1           /* state and fluxes FP32; dt, cumulative infiltration and budgets FP64 */
+};
+
+/* Field ids for swehr_download().  k is the 1-based size class for per-class fields. */
+enum swehr_field {
+    SWEHR_F_DEPTH = 0,   /* U[1]  h (m)                         depth.txt   */
+    SWEHR_F_UH = 1,      /* U[2]  h*u (m^2/s)                   uh.txt      */
+    SWEHR_F_VH = 2,      /* U[3]  h*v (m^2/s)                   vh.txt      */
+    SWEHR_F_CH = 3,      /* U[3+k] suspended mass (kg/m^2)      ch.txt (k=1)*/
+    SWEHR_F_M = 4,       /* M[k]  deposited mass (kg/m^2)                   */
+    SWEHR_F_TOPO = 5,    /* TOPO  bed elevation (m)             topoout.txt */
+    SWEHR_F_VINF = 6,    /* VINF  cumulative infiltration (m)               */
+    SWEHR_F_TOTM = 7,    /* TOTM  deposited mass, all classes   m.txt       */
+    SWEHR_F_DEP = 8,     /* DEP   deposited thickness (m)                   */
+    SWEHR_F_TOTSED = 9,  /* TOTSED volumetric concentration     c.txt       */
+    SWEHR_F_SPEED = 10,  /* |VEL| (m/s)                         vel.txt     */
+    SWEHR_F_MAXVEL = 11, /* running max |VEL|                   maxvel.txt  */
+    SWEHR_F_MAXDEPTH = 12,/* running max h                      maxdepth.txt*/
+    SWEHR_F_SED = 13,    /* SED[k] concentration (kg/m^3)                   */
+    SWEHR_F_VELX = 14,   /* VEL[1] (m/s)                                    */
+    SWEHR_F_VELY = 15,   /* VEL[2] (m/s)                                    */
+    SWEHR_F_COUNT = 16
+};
+
+/*
+ * Run parameters.  The first block is input.txt (SWEHR.c:474-504, field number in
+ * brackets); the second block holds values the reference hard-codes, exposed so the
+ * optional sidecar file can set them.  swehr_params_init() fills the reference values.
+ */
+typedef struct swehr_params {
+    int nx;            /* [1] cells across slope (i)                               */
+    int ny;            /* [2] cells downslope (j); j=1 is the outlet row            */
+    double dx;         /* [3] grid spacing (m)                                      */
+    double F;          /* [4] effective fraction of stream power in entrainment     */
+    double manning0;   /* [5] read and ignored, as in the reference (SWEHR.c:480)  */
+    double tend;       /* [6] simulated time (s)                                    */
+    double a;          /* [7] rainsplash detachability of original soil            */
+    double ad;         /* [8] rainsplash detachability of deposited sediment       */
+    double h0;         /* [9] critical depth for rainsplash                         */
+    double mtstar0;    /* [10] deposited mass that shields the soil                 */
+    int rnum;          /* [11] rain table entries                                   */
+    double rint;       /* [12] seconds between rain entries                         */
+    int nclass;        /* [13] size classes S (1..SWEHR_MAX_CLASSES)                */
+    double J;          /* [14] energy to entrain a unit mass                        */
+    /* hard-coded in the reference */
+    double cstable;    /* 0.05  target courant number   (SWEHR.c:632) */
+    double maxstep;    /* 0.25  largest dt (s)          (SWEHR.c:633) */
+    int bndry[4];      /* {1,0,1,1} i=1, j=1, i=nx, j=ny: 1 solid, 0 transmissive (SWEHR.c:617-621) */
+    int inflow;        /* 1     water+sediment inflow at j=ny (SWEHR.c:1010-1035) */
+    double roughness;  /* 0.1   Manning n               (SWEHR.c:237) */
+    double theta0;     /* 0.05  initial soil moisture   (SWEHR.c:240) */
+    double thetas;     /* 0.4   saturated soil moisture (SWEHR.c:241) */
+    double hf;         /* 0.001 Green-Ampt suction head (SWEHR.c:242) */
+    double ks;         /* 0     saturated conductivity (m/s) when no raster (SWEHR.c:239) */
+    double vinf0;      /* 1e-4  initial cumulative infiltration (SWEHR.c:243) */
+} swehr_params;
+
+/* State after the last step taken by swehr_advance(). */
+typedef struct swehr_step_info {
+    double t;          /* simulated time (s)                                  */
+    double dt;         /* dt of the last step                                  */
+    double courant;    /* Smax*dt/dx of the last step (SWEHR.c:1412)           */
+    double rain;       /* rain rate R of the last step (m/s)                   */
+    int64_t steps;     /* steps taken since create                             */
+    int32_t done;      /* 1 once t >= tend or the run stopped                  */
+    int32_t print_due; /* 1: the last step crossed a print interval (SWEHR.c:1679); */
+                       /*    the run pauses until swehr_ack_print()            */
+    int32_t stage_count; /* outlet/probe records so far                        */
+    int32_t status;    /* enum swehr_status of the run                          */
+} swehr_step_info;
+
+/* Mass-budget terms (FP64, compensated sums over all cells), in m^3 of water and kg of
+ * sediment.  index: see swehr_budget(). */
+#define SWEHR_BUDGET_N 12
+
+typedef struct swehr_handle swehr_handle;
+
+/* Fill *p with the reference's defaults (input fields zero). */
+int swehr_params_init(swehr_params *p);
+
+/*
+ * Create a solver instance on the current GPU and upload the initial condition the
+ * reference sets up (SWEHR.c:211-325, 537-803): bed from topo, zero depth, momentum
+ * and sediment, VINF=vinf0, lagged geometry seeded from topo.
+ *   topo   nx*ny bed elevations, i-major (topoin.txt order)
+ *   ks     nx*ny saturated conductivity raster (m/s), or NULL for the scalar p->ks
+ *   ds     nclass particle diameters (m)           (particlesizein.txt)
+ *   pfrac  nclass fractions in the original soil   (particlepercentin.txt)
+ *   rain   rnum rain rates (m/s)                    (rain.txt)
+ *   precision  enum swehr_precision
+ */
+int swehr_create(const swehr_params *p, const double *topo, const double *ks,
+                 const double *ds, const double *pfrac, const double *rain,
+                 int precision, swehr_handle **out);
+
+/*
+ * Take up to max_steps time steps (each: boundary fill, fluxes and sources, CFL dt,
+ * update, outlet sampling).  Stops early when t >= tend, when a print interval is
+ * crossed (info->print_due; call swehr_ack_print() to continue) or on error.
+ * Steps run as captured HIP graphs; the host reads back only *info.
+ */
+int swehr_advance(swehr_handle *h, int64_t max_steps, swehr_step_info *info);
+
+/* Clear the print pause raised by swehr_advance(). */
+int swehr_ack_print(swehr_handle *h);
+
+/* Copy one field (enum swehr_field; k = 1-based class for per-class fields) to a host
+ * raster of nx*ny doubles, i-major. */
+int swehr_download(swehr_handle *h, int field, int k, double *host);
+
+/* Copy the outlet series: stage[c*capacity + r] for column c < SWEHR_STAGE_COLS and
+ * saveflow[c*capacity + r] for c < SWEHR_FLOW_COLS, r < *count (SWEHR.c:1608-1675). */
+int swehr_stage(swehr_handle *h, double *stage, double *saveflow, int capacity, int *count);
+
+/* Global budgets, FP64 compensated sums:
+ *  [0] water on the surface  sum h*dx^2            [1] water infiltrated sum (VINF-vinf0)*dx^2
+ *  [2] rain added            sum dt*R*dx^2 (cells)  [3] net boundary water outflow (m^3)
+ *  [4] suspended sediment    sum ch*dx^2 (kg)      [5] deposited sediment sum M*dx^2 (kg)
+ *  [6] bed change            sum (z-z0)*dx^2 (m^3) [7] net boundary sediment outflow (kg)
+ *  [8] sediment detached from the original soil (kg) [9] water added by film clamps (m^3)
+ *  [10] water from sediment-volume exchange (m^3)  [11] steps accumulated */
+int swehr_budget(swehr_handle *h, double *out);
+
+/* Device bytes held by one instance of these sizes (for admission control). */
+int64_t swehr_instance_bytes(int nx, int ny, int nclass, int precision);
+
+/* Release the instance and its device memory. */
+int swehr_destroy(swehr_handle *h);
+
+/* Text for a status code. */
+const char *swehr_strerror(int code);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SWEHR_H */

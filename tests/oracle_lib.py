@@ -1,0 +1,116 @@
+"""TEST INFRASTRUCTURE: ctypes access to the CPU oracle (oracle/liboracle.so).
+
+The oracle is the checker: tests compare the HIP path with it, never the other way
+round, and the product never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from swehr_amd import _abi
+from swehr_amd._abi import dptr
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+RUN_PATH = os.path.join(ROOT, "oracle", "_bin", "swehr_oracle_run")
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        L.oracle_create.restype = C.c_void_p
+        L.oracle_create.argtypes = [C.POINTER(_abi.Params)] + [C.POINTER(C.c_double)] * 5
+        L.oracle_step.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.oracle_run.argtypes = [C.c_void_p, C.c_long]
+        L.oracle_info.argtypes = [C.c_void_p, C.POINTER(_abi.StepInfo)]
+        L.oracle_download.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]
+        L.oracle_stage.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int,
+                                   C.POINTER(C.c_int)]
+        L.oracle_budget.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        L.oracle_fail_events.restype = C.c_long
+        L.oracle_fail_events.argtypes = [C.c_void_p]
+        L.oracle_set_threads.argtypes = [C.c_int]
+        L.oracle_destroy.argtypes = [C.c_void_p]
+        _lib = L
+    return _lib
+
+
+class Oracle:
+    """One oracle instance for a swehr_amd.inputs.Case (FP64)."""
+
+    def __init__(self, case, threads: int = 1):
+        L = lib()
+        L.oracle_set_threads(int(threads))
+        self.case = case
+        self._p = _abi.params_from_case(case)
+        topo, ks, ds, pf, rain = _abi.case_arrays(case)
+        self._keep = (topo, ks, ds, pf, rain)
+        h = L.oracle_create(C.byref(self._p), dptr(topo), dptr(ks) if ks is not None else None, dptr(ds),
+                            dptr(pf), dptr(rain))
+        if not h:
+            raise ValueError("oracle_create rejected the case")
+        self._h = h
+
+    def step(self) -> int:
+        pd = C.c_int(0)
+        return lib().oracle_step(self._h, C.byref(pd))
+
+    def run(self, n: int = 1 << 62) -> int:
+        return lib().oracle_run(self._h, int(n))
+
+    def info(self) -> dict:
+        i = _abi.StepInfo()
+        lib().oracle_info(self._h, C.byref(i))
+        return i.as_dict()
+
+    def field(self, name: str, k: int = 0) -> np.ndarray:
+        out = np.empty(self.case.nx * self.case.ny, dtype=np.float64)
+        rc = lib().oracle_download(self._h, _abi.FIELDS[name], int(k), dptr(out))
+        if rc != 0:
+            raise ValueError("oracle_download(%s, %d) -> %d" % (name, k, rc))
+        return out.reshape(self.case.nx, self.case.ny)
+
+    def stage(self):
+        cap = _abi.STAGE_MAX
+        st = np.zeros(_abi.STAGE_COLS * cap)
+        fl = np.zeros(_abi.FLOW_COLS * cap)
+        cnt = C.c_int(0)
+        lib().oracle_stage(self._h, dptr(st), dptr(fl), cap, C.byref(cnt))
+        n = cnt.value
+        return st.reshape(_abi.STAGE_COLS, cap)[:, :n].copy(), fl.reshape(_abi.FLOW_COLS, cap)[:, :n].copy()
+
+    def budget(self) -> np.ndarray:
+        out = np.zeros(_abi.BUDGET_N)
+        lib().oracle_budget(self._h, dptr(out))
+        return out
+
+    def fail_events(self) -> int:
+        return int(lib().oracle_fail_events(self._h))
+
+    def state(self) -> dict:
+        """All persistent fields as (nx, ny) arrays; per-class fields as (S, nx, ny)."""
+        S = self.case.nclass
+        d = {k: self.field(k) for k in ("depth", "uh", "vh", "topo", "vinf", "totm", "dep", "totsed",
+                                        "maxvel", "maxdepth", "velx", "vely")}
+        d["ch"] = np.stack([self.field("ch", k + 1) for k in range(S)])
+        d["m"] = np.stack([self.field("m", k + 1) for k in range(S)])
+        return d
+
+    def close(self):
+        if self._h:
+            lib().oracle_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
