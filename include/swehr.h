@@ -169,6 +169,30 @@ int swehr_stage(swehr_handle *h, double *stage, double *saveflow, int capacity, 
  *  [10] water from sediment-volume exchange (m^3)  [11] steps accumulated */
 int swehr_budget(swehr_handle *h, double *out);
 
+/* Run-loop options.  pause_on_print: 1 (default) pause swehr_advance() at each print
+ * interval so the caller can write the reference's progress line and movie frame
+ * (SWEHR.c:1679-1698); 0 never pause.  steps_per_graph: time steps captured per HIP
+ * graph launch (default 16; <= 0 keeps the current value).  tile: kernel tile shape
+ * id (0 = automatic choice; see swehr_tile_count()). */
+int swehr_configure(swehr_handle *h, int pause_on_print, int steps_per_graph, int tile);
+
+/* Number of tile shapes the step kernel is built for, and the shape of id (1-based):
+ * cells across (i) and downslope (j) per workgroup. */
+int swehr_tile_count(void);
+int swehr_tile_shape(int id, int *ti, int *tj);
+
+/* The HIP stream (hipStream_t) the instance's kernels run on, for event timing. */
+int swehr_get_stream(swehr_handle *h, void **stream);
+
+/* Diagnostic, no counterpart in the reference: take up to nsteps steps launched one by
+ * one (not as a graph) with HIP events around every kernel on the instance's stream,
+ * and return the average duration in milliseconds of the wave-speed, step and control
+ * kernels in avg_ms[0..2]; *taken receives the number of steps run. */
+int swehr_profile(swehr_handle *h, int nsteps, double *avg_ms, int *taken);
+
+/* Last error message of the calling thread (empty when none). */
+const char *swehr_last_error(void);
+
 /* Device bytes held by one instance of these sizes (for admission control). */
 int64_t swehr_instance_bytes(int nx, int ny, int nclass, int precision);
 

@@ -1,0 +1,714 @@
+// swehr_device.h - device-side data structures and the three per-step kernels of the
+// SWEHR coupled time step on gfx950.  Shared by the single-instance library
+// (swehr_lib.hip) and the batched/serving runners (swehr_batch.hip).
+//
+// Memory layout in HBM (one instance): structure of arrays, one plane per variable,
+// value (i, j) at [i * py + j] with j (downslope) unit-stride and the row pitch py
+// rounded up to 64 elements, so every row starts on a 256 B (FP64) / 128 B (FP32)
+// boundary and a wavefront reads a row segment in whole cache lines.  Fields read at
+// neighbours (h, hu, hv, ch_k, DEP) are double-buffered by step parity; the bed z has
+// three buffers (previous, current, next) because the reference reads geometry one
+// step late (SWEHR.c:1135, 1243, 1293, 1304); fields read only at their own cell
+// (M_k, TOTM, VINF, MAXVEL, MAXDEPTH) are updated in place.  SED, TOTSED, RHO and VEL
+// are not stored: they are exact functions of (h, ch_k, hu, hv) (SWEHR.c:1516-1531,
+// 1563-1564, 1578-1579) and are recomputed where needed.
+//
+// One step = three launches, all guarded by the device control block so that a
+// captured graph of many steps stops by itself at t >= tend, at a print pause, or at
+// a step limit:
+//   K1 wave_speed_kernel  max |wave speed| over every face (SWEHR.c:1161, 1228)
+//                         -> atomicMax on the bit pattern of a non-negative double
+//   K2 step_kernel        boundary ghosts, fluxes, sources, dt, update (SWEHR.c:816-1589)
+//                         on LDS-staged tiles; every face flux computed once per tile
+//   K3 control_kernel     t += dt, outlet and probe records, print pause (SWEHR.c:1591-1682)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "swehr_physics.h"
+
+namespace swehr {
+
+constexpr int kMaxClasses = 16;
+constexpr int kStageCols = 24;
+constexpr int kFlowCols = 18;
+constexpr int kStageMax = 30000;
+
+// per-block budget terms accumulated across steps (sum and Neumaier compensation)
+enum BudgetTerm { B_EXCH = 0, B_CLAMP = 1, B_DET = 2, B_BWATER = 3, B_BSED = 4, NB_TERMS = 5 };
+
+struct alignas(64) DevCtrl {
+    double t, dt, courant, rain;
+    double stageinterval, printinterval, printint, rain_sum;
+    double rain_comp;
+    unsigned long long smax_bits;
+    long long steps, step_limit;
+    int stagecnt, status, done, print_due;
+    int pause_on_print, pad[3];
+};
+
+template <typename R>
+struct StepArgs {
+    int nx, ny, S, py;
+    long long plane;
+    R *h[2], *hu[2], *hv[2], *ch[2], *dep[2];
+    R *z[3];
+    R *M, *totm, *maxvel, *maxdepth;
+    double *vinf;
+    const double *ks_raster;  // per-cell Ks (pitched like the state) or null
+    const double *rain;
+    double ks, dx, h0, mtstar0, a, ad, F, J, roughness, theta0, thetas, hf, cstable, maxstep, tend, rint, vinf0;
+    int rnum, inflow, bndry[4];
+    int tiles_i, tiles_j, nblocks, blocks_per_xcd;
+    int bud_slots;   // budget slots allocated (>= nblocks of every tile shape)
+    int stage_cap;   // rows of the stage / saveflow series
+    double ds[kMaxClasses], pf[kMaxClasses];
+    DevCtrl *ctrl;
+    double *bud;       // [nblocks][NB_TERMS][2]
+    double *stage;     // [kStageCols][stage_cap]
+    double *saveflow;  // [kFlowCols][stage_cap]
+};
+
+__device__ __forceinline__ bool step_active(const DevCtrl *C)
+{
+    return !C->done && !C->print_due && C->steps < C->step_limit;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        const T w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ void neumaier(double &s, double &c, double x)
+{
+    const double t = s + x;
+    if (fabs(s) >= fabs(x)) c += (s - t) + x;
+    else c += (x - t) + s;
+    s = t;
+}
+
+// One-sided at the domain edges exactly as SWEHR.c:1240-1277; P is an LDS plane with
+// row stride ZJ holding z (or the previous step's z) around the cell at (zi, zj).
+template <typename R, int ZJ>
+__device__ __forceinline__ R slope_x(const R *P, int zi, int zj, int gi, int nx, R odx)
+{
+    if (gi != 0 && gi != nx - 1) return R(0.5) * odx * (P[(zi + 1) * ZJ + zj] - P[(zi - 1) * ZJ + zj]);
+    if (gi == nx - 1) return odx * (P[zi * ZJ + zj] - P[(zi - 1) * ZJ + zj]);
+    return odx * (P[(zi + 1) * ZJ + zj] - P[zi * ZJ + zj]);
+}
+
+template <typename R, int ZJ>
+__device__ __forceinline__ R slope_y(const R *P, int zi, int zj, int gj, int ny, R odx)
+{
+    if (gj != 0 && gj != ny - 1) return R(0.5) * odx * (P[zi * ZJ + zj + 1] - P[zi * ZJ + zj - 1]);
+    if (gj == ny - 1) return odx * (P[zi * ZJ + zj] - P[zi * ZJ + zj - 1]);
+    return odx * (P[zi * ZJ + zj + 1] - P[zi * ZJ + zj]);
+}
+
+// cos(atan|x|) = 1 / sqrt(1 + x^2)
+template <typename R>
+__device__ __forceinline__ R cos_atan(R x)
+{
+    return R(1) / sqrt(R(1) + x * x);
+}
+
+// --------------------------------------------------------------------------- K1
+
+// Global-memory slope for the wave-speed pass (no LDS): z at clamped neighbours.
+template <typename R>
+__device__ __forceinline__ R gz_from_plane(const R *zp, int i, int j, int nx, int ny, int py, R odx)
+{
+    const long long c = (long long)i * py + j;
+    R sx, sy;
+    if (i != 0 && i != nx - 1) sx = R(0.5) * odx * (zp[c + py] - zp[c - py]);
+    else if (i == nx - 1) sx = odx * (zp[c] - zp[i > 0 ? c - py : c]);
+    else sx = odx * (zp[nx > 1 ? c + py : c] - zp[c]);
+    if (j != 0 && j != ny - 1) sy = R(0.5) * odx * (zp[c + 1] - zp[c - 1]);
+    else if (j == ny - 1) sy = odx * (zp[c] - zp[j > 0 ? c - 1 : c]);
+    else sy = odx * (zp[ny > 1 ? c + 1 : c] - zp[c]);
+    return R(kG) / sqrt(sx * sx + sy * sy + R(1));
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void wave_speed_kernel(StepArgs<R> A)
+{
+    DevCtrl *C = A.ctrl;
+    if (!step_active(C)) return;
+    const long long steps = C->steps;
+    const int p = (int)(steps & 1), p3 = (int)(steps % 3);
+    const R *h = A.h[p], *hu = A.hu[p], *hv = A.hv[p];
+    const R *zp = A.z[(p3 + 2) % 3];
+    const int nx = A.nx, ny = A.ny, py = A.py;
+    const R odx = R(1.0 / A.dx);
+    const double t = C->t;
+    InflowState in = inflow_state(t);
+    R smax = R(0);
+    const long long n = (long long)nx * ny;
+    for (long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x; cell < n;
+         cell += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(cell / ny), j = (int)(cell % ny);
+        const long long c = (long long)i * py + j;
+        const R g = gz_from_plane(zp, i, j, nx, ny, py, odx);
+        const R hc = h[c], huc = hu[c], hvc = hv[c];
+        // face i+1/2 (right ghost at i = nx-1)
+        {
+            R hr, qr;
+            if (i < nx - 1) { hr = h[c + py]; qr = hu[c + py]; }
+            else { hr = hc; qr = A.bndry[2] == 1 ? -huc : huc; }
+            const R s = face_speed(hc, hr, huc, qr, g);
+            smax = s > smax ? s : smax;
+        }
+        // face j+1/2 (top ghost at j = ny-1: inflow or side rule)
+        {
+            R hr, qr;
+            if (j < ny - 1) { hr = h[c + 1]; qr = hv[c + 1]; }
+            else if (A.inflow) { hr = R(in.h); qr = R(in.hv); }
+            else { hr = hc; qr = A.bndry[3] == 1 ? -hvc : hvc; }
+            const R s = face_speed(hc, hr, hvc, qr, g);
+            smax = s > smax ? s : smax;
+        }
+        if (i == 0) {  // face 1/2: ghost on the left, GZ of cell 1 (SWEHR.c:1054-1059)
+            const R hl = hc, ql = A.bndry[0] == 1 ? -huc : huc;
+            const R s = face_speed(hl, hc, ql, huc, g);
+            smax = s > smax ? s : smax;
+        }
+        if (j == 0) {
+            const R hl = hc, ql = A.bndry[1] == 1 ? -hvc : hvc;
+            const R s = face_speed(hl, hc, ql, hvc, g);
+            smax = s > smax ? s : smax;
+        }
+    }
+    double m = wave_max((double)smax);
+    __shared__ double red[16];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = red[w] > m ? red[w] : m;
+        atomicMax(&C->smax_bits, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+// --------------------------------------------------------------------------- K2
+
+template <typename R, int TI, int TJ>
+struct TileGeom {
+    static constexpr int NT = TI * TJ;
+    static constexpr int EI = TI + 2, EJ = TJ + 2, EXT = EI * EJ;  // 1-cell halo
+    static constexpr int ZI = TI + 4, ZJ = TJ + 4, ZEXT = ZI * ZJ;  // 2-cell halo (bed)
+    static constexpr int NFX = (TI + 1) * TJ, NFY = TI * (TJ + 1);
+    static size_t lds_bytes(int S)
+    {
+        return sizeof(double) * NB_TERMS * 16 + sizeof(R) * ((size_t)(5 + S) * EXT + 2 * ZEXT + 3 * NFX + 3 * NFY);
+    }
+};
+
+template <typename R, int TI, int TJ>
+__global__ __launch_bounds__(TI *TJ) void step_kernel(StepArgs<R> A)
+{
+    using T = TileGeom<R, TI, TJ>;
+    constexpr int NT = T::NT, EJ = T::EJ, EXT = T::EXT, ZI = T::ZI, ZJ = T::ZJ, ZEXT = T::ZEXT;
+    constexpr int NFX = T::NFX, NFY = T::NFY;
+    DevCtrl *C = A.ctrl;
+    if (!step_active(C)) return;
+
+    // XCD-aware tile order: consecutive block ids land on different XCDs, so give
+    // each XCD a contiguous band of tiles (their halos then share that XCD's L2).
+    const int b = blockIdx.x;
+    const int tile = (b & 7) * A.blocks_per_xcd + (b >> 3);
+    if (tile >= A.nblocks) return;
+    const int ti = tile / A.tiles_j, tj = tile % A.tiles_j;
+    const int i0 = ti * TI, j0 = tj * TJ;
+
+    const long long steps = C->steps;
+    const double t = C->t;
+    const double smax = __longlong_as_double((long long)C->smax_bits);
+    const double dtd = step_dt(smax, A.cstable, A.dx, A.maxstep, t, A.tend);
+    const double Rd = rain_rate(A.rain, A.rnum, A.rint, t);
+    const int p = (int)(steps & 1), q = p ^ 1, p3 = (int)(steps % 3);
+    const int nx = A.nx, ny = A.ny, py = A.py, S = A.S;
+    const long long plane = A.plane;
+    const R *hin = A.h[p], *huin = A.hu[p], *hvin = A.hv[p], *chin = A.ch[p], *depin = A.dep[p];
+    const R *zc = A.z[p3], *zp = A.z[(p3 + 2) % 3];
+    R *zn = A.z[(p3 + 1) % 3];
+
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double *s_red = reinterpret_cast<double *>(smem_raw);
+    R *s_h = reinterpret_cast<R *>(s_red + NB_TERMS * 16);
+    R *s_hu = s_h + EXT, *s_hv = s_hu + EXT, *s_ts = s_hv + EXT, *s_dep = s_ts + EXT;
+    R *s_z = s_dep + EXT, *s_zp = s_z + ZEXT;
+    R *s_fx = s_zp + ZEXT, *s_fy = s_fx + 3 * NFX;
+    R *s_sed = s_fy + 3 * NFY;
+
+    const R hfilm = R(kHfilm);
+    const R rhos = R(kRhos), rhow = R(kRhow);
+    const int tid = threadIdx.x;
+
+    // ---- phase 1: ghosts and the 1-cell halo of the conserved state (SWEHR.c:816-1035)
+    const InflowState in = inflow_state(t);
+    for (int e = tid; e < EXT; e += NT) {
+        const int ei = e / EJ, ej = e - (e / EJ) * EJ;
+        const int gi = i0 - 1 + ei, gj = j0 - 1 + ej;
+        const int ci = gi < 0 ? 0 : (gi >= nx ? nx - 1 : gi);
+        const int cj = gj < 0 ? 0 : (gj >= ny ? ny - 1 : gj);
+        const long long c = (long long)ci * py + cj;
+        R h = hin[c], hu = huin[c], hv = hvin[c];
+        if (gi < 0 && A.bndry[0] == 1) hu = -hu;
+        if (gi >= nx && A.bndry[2] == 1) hu = -hu;
+        if (gj < 0 && A.bndry[1] == 1) hv = -hv;
+        const bool inflow_ghost = gj >= ny && A.inflow;
+        if (gj >= ny && !A.inflow && A.bndry[3] == 1) hv = -hv;
+        R ts = R(0);
+        if (inflow_ghost) {
+            h = R(in.h);
+            hu = R(0);
+            hv = R(in.hv);
+            const R cc = R(in.c);
+            for (int k = 0; k < S; k++) s_sed[k * EXT + e] = cc / h;
+        } else {
+            const bool wet = h > hfilm;
+            for (int k = 0; k < S; k++) {
+                R sd = R(0);
+                if (wet) {
+                    sd = chin[k * plane + c] / h;
+                    ts += sd / rhos;
+                    if (sd < R(-1e-5)) sd = R(0);
+                }
+                s_sed[k * EXT + e] = sd;
+            }
+        }
+        s_h[e] = h;
+        s_hu[e] = hu;
+        s_hv[e] = hv;
+        s_ts[e] = ts;
+        s_dep[e] = depin[c];
+    }
+    for (int e = tid; e < ZEXT; e += NT) {
+        const int zi = e / ZJ, zj = e - (e / ZJ) * ZJ;
+        int gi = i0 - 2 + zi, gj = j0 - 2 + zj;
+        gi = gi < 0 ? 0 : (gi >= nx ? nx - 1 : gi);
+        gj = gj < 0 ? 0 : (gj >= ny ? ny - 1 : gj);
+        const long long c = (long long)gi * py + gj;
+        s_z[e] = zc[c];
+        s_zp[e] = zp[c];
+    }
+    __syncthreads();
+
+    const R odx = R(1.0 / A.dx);
+    const R g = R(kG);
+    // GZ of the previous step's flux pass at global cell (gi, gj) (SWEHR.c:1279, read
+    // lagged at 1135 and 1203); gi, gj are inside the domain.
+    auto gz_lag = [&](int gi, int gj) -> R {
+        const int zi = gi - (i0 - 2), zj = gj - (j0 - 2);
+        const R a = slope_x<R, ZJ>(s_zp, zi, zj, gi, nx, odx);
+        const R bb = slope_y<R, ZJ>(s_zp, zi, zj, gj, ny, odx);
+        return g / sqrt(a * a + bb * bb + R(1));
+    };
+
+    // ---- phase 2: face fluxes of the tile, each computed once
+    const int il = tid / TJ, jl = tid - (tid / TJ) * TJ;
+    auto xface = [&](int fi, int fj) {
+        const int gil = i0 + fi - 1, gj = j0 + fj;
+        if (gj >= ny || gil >= nx) return;
+        const int el = fi * EJ + fj + 1, er = el + EJ;
+        const R gz = gz_lag(gil < 0 ? 0 : gil, gj);
+        R f0, f1, f2;
+        face_flux(s_h[el], s_h[er], s_hu[el], s_hu[er], s_hv[el], s_hv[er], gz, f0, f1, f2);
+        const int f = fi * TJ + fj;
+        s_fx[f] = f0;
+        s_fx[NFX + f] = f1;
+        s_fx[2 * NFX + f] = f2;
+    };
+    auto yface = [&](int fi, int fj) {
+        const int gi = i0 + fi, gjl = j0 + fj - 1;
+        if (gi >= nx || gjl >= ny) return;
+        const int el = (fi + 1) * EJ + fj, er = el + 1;
+        const R gz = gz_lag(gi, gjl < 0 ? 0 : gjl);
+        R f0, f1, f2;
+        face_flux(s_h[el], s_h[er], s_hv[el], s_hv[er], s_hu[el], s_hu[er], gz, f0, f1, f2);
+        const int f = fi * (TJ + 1) + fj;
+        s_fy[f] = f0;
+        s_fy[NFY + f] = f2;      // hu flux (tangential)
+        s_fy[2 * NFY + f] = f1;  // hv flux (normal), SWEHR.c:1216-1218
+    };
+    xface(il + 1, jl);
+    yface(il, jl + 1);
+    if (il == 0) xface(0, jl);
+    if (jl == 0) yface(il, 0);
+    __syncthreads();
+
+    // ---- phase 3: sources and update of the tile's own cells
+    double bud[NB_TERMS] = {0, 0, 0, 0, 0};
+    const int gi = i0 + il, gj = j0 + jl;
+    if (gi < nx && gj < ny) {
+        const long long c = (long long)gi * py + gj;
+        const int e = (il + 1) * EJ + (jl + 1);
+        const int zi = il + 2, zj = jl + 2;
+        const R h = s_h[e], hu = s_hu[e], hv = s_hv[e];
+        const R ts = s_ts[e];
+        const R dt = R(dtd), Rr = R(Rd);
+        const R h0 = R(A.h0);
+
+        // geometry of this step; GX takes SLOPEY of the previous step (SWEHR.c:1239-1280)
+        const R sxn = slope_x<R, ZJ>(s_z, zi, zj, gi, nx, odx);
+        const R syo = slope_y<R, ZJ>(s_zp, zi, zj, gj, ny, odx);
+        const R syn = slope_y<R, ZJ>(s_z, zi, zj, gj, ny, odx);
+        const R gx = g * sxn / sqrt(sxn * sxn + syo * syo + R(1));
+        const R sxs = -gx * h;
+        const R nrm = sqrt(sxn * sxn + syn * syn + R(1));
+        const R gy = g * syn / nrm;
+        const R sys = -gy * h;
+        const R gz = g / nrm;
+
+        // mixture density from the previous update (SWEHR.c:1578-1579)
+        R rho = rhow * (R(1) - ts) + rhos * ts;
+        if (rho > R(0.8) * rhos) rho = R(0.8) * rhos;
+
+        // variable-density forcing (SWEHR.c:1282-1290)
+        R gmx = R(0), gmy = R(0);
+        if (gi > 0 && gi < nx - 1) gmx -= (rhos - rho) * gz * h * h / (R(2) * rho) * R(.5) * odx * (s_ts[e + EJ] - s_ts[e - EJ]);
+        if (gj > 0 && gj < ny - 1) gmy -= (rhos - rho) * gz * h * h / (R(2) * rho) * R(.5) * odx * (s_ts[e + 1] - s_ts[e - 1]);
+
+        // stability of the deposited layer (SWEHR.c:1292-1312): right/upper neighbour
+        // slopes of the previous step, left/lower ones of this step, own at the edges
+        const R rho0 = R(kRhow * kPhi + kRhos * (1 - kPhi));
+        const int du = gi < nx - 1 ? 1 : 0, dd = gi > 0 ? 1 : 0;
+        const int ju = gj < ny - 1 ? 1 : 0, jd = gj > 0 ? 1 : 0;
+        const R sx_up = du ? slope_x<R, ZJ>(s_zp, zi + 1, zj, gi + 1, nx, odx) : sxn;
+        const R sx_dn = dd ? slope_x<R, ZJ>(s_z, zi - 1, zj, gi - 1, nx, odx) : sxn;
+        const R sy_up = ju ? slope_y<R, ZJ>(s_zp, zi, zj + 1, gj + 1, ny, odx) : syn;
+        const R sy_dn = jd ? slope_y<R, ZJ>(s_z, zi, zj - 1, gj - 1, ny, odx) : syn;
+        const R dep = s_dep[e];
+        const R cy = cos_atan(syn);
+        const R load = rho0 * gz * dep + rho * gz * h;
+        const R lhs = load * (fabs(syn) * cy);
+        const R base = load * cy - rhow * gz * (dep + h);
+        const R kfa = R(kFrictionAngle), kco = R(kCohesion);
+        R pgfx = dep * rho0 * gz * R(0.5) * odx *
+                 (s_dep[e + du * EJ] * cos_atan(sx_up) - s_dep[e - dd * EJ] * cos_atan(sx_dn));
+        if (pgfx * sxn > R(0)) pgfx = R(-1) * fabs(pgfx);
+        if (pgfx * sxn < R(0)) pgfx = fabs(pgfx);
+        bool fail = lhs > (kco + base * kfa + pgfx);
+        if (!fail) {
+            R pgfy = dep * rho0 * gz * R(0.5) * odx *
+                     (s_dep[e + ju] * cos_atan(sy_up) - s_dep[e - jd] * cos_atan(sy_dn));
+            if (pgfy * syn > R(0)) pgfy = R(-1) * fabs(pgfy);
+            if (pgfy * syn < R(0)) pgfy = fabs(pgfy);
+            fail = lhs > (kco + base * kfa + pgfy);
+        }
+
+        // debris-flow forcing (SWEHR.c:1314-1320)
+        R dfm = fmin(R(1), (ts - R(0.2)) / R(0.2));
+        if (h < R(9) * h0 || ts < R(0.2)) dfm = R(0);
+        const R bt = -(R(1) - R(kLambda)) * gz * h * kfa;
+        const R btx = hu < R(0) ? R(-1) * bt : bt;
+        const R bty = hv < R(0) ? R(-1) * bt : bt;
+
+        // class-invariant parts of the Hairsine-Rose sources (SWEHR.c:1329-1360)
+        const R totm_old = A.totm[c];
+        const R hvert = h * nrm;
+        const bool deep_v = hvert > h0;
+        const R mtstar = deep_v ? R(A.mtstar0) * (h0 / hvert) : R(A.mtstar0);
+        const R H = fmin(R(1), totm_old / mtstar);
+        const R shield = deep_v ? h0 / hvert : R(1);
+        const R rough = R(A.roughness);
+        const R hc = R(kHcFriction);
+        const R manning = (h <= hc) ? rough * R(pow(h / hc, R(kDepthExp))) : rough;
+        const R qq = hu * hu + hv * hv;
+        const R qm = sqrt(qq);
+        const R sf = manning * manning * qm * qm / R(pow(h, R(3.333)));
+        const R omega = rho * gz * sf * qm;
+        const bool active = h > R(10) * hfilm && ts < R(0.6);
+        const R oneh = R(1) - fmin(ts, R(0.99));
+        const R hind = (oneh * oneh) * (oneh * oneh);
+        const bool shutoff = h > R(9) * h0 && ts > R(kCthreshold);
+        const bool shallow_v = h < R(9) * h0;
+        const R vshal = fmax(h - R(3) * h0, R(0));
+        const R s1 = R(kRhos / kRhow - 1);
+        const R cg = R(1.09) * s1 * gz;
+        const R e4den = (rhos - rho) / rhos * gz * h;
+
+        // conservative update (SWEHR.c:1424-1441)
+        const int fxl = il * TJ + jl, fxr = fxl + TJ;
+        const int fyl = il * (TJ + 1) + jl, fyr = fyl + 1;
+        const R dx_ = dt * odx;
+        R hn = h - dx_ * (s_fx[fxr] - s_fx[fxl]) - dx_ * (s_fy[fyr] - s_fy[fyl]) + dt * Rr;
+        R hun = hu - dx_ * (s_fx[NFX + fxr] - s_fx[NFX + fxl]) - dx_ * (s_fy[NFY + fyr] - s_fy[NFY + fyl]) + dt * sxs +
+                dt * gmx + dt * dfm * btx;
+        R hvn = hv - dx_ * (s_fx[2 * NFX + fxr] - s_fx[2 * NFX + fxl]) -
+                dx_ * (s_fy[2 * NFY + fyr] - s_fy[2 * NFY + fyl]) + dt * sys + dt * gmy + dt * dfm * bty;
+
+        // Green-Ampt infiltration in FP64; negative below the film depth (SWEHR.c:1448-1452)
+        {
+            const double ks = A.ks_raster ? A.ks_raster[c] : A.ks;
+            const double vin = A.vinf[c];
+            const double zf = vin / (A.thetas - A.theta0);
+            const double hd = (double)hn;
+            const double infl = fmin(hd - kHfilm, dtd * ks * (zf + A.hf + hd) / zf);
+            A.vinf[c] = vin + infl;
+            hn = R(hd - infl);
+        }
+
+        // sediment: flux divergence, sources and exchange per class (SWEHR.c:1454-1511)
+        const R fx0l = s_fx[fxl], fx0r = s_fx[fxr], fy0l = s_fy[fyl], fy0r = s_fy[fyr];
+        const int exl = e - EJ, exr = e + EJ, eyl = e - 1, eyr = e + 1;
+        const R kbed = R(1) / (rhos * (R(1) - R(kPhi)));
+        const R rsp = rhos * (R(1) - R(kPhi));
+        R zn_c = s_z[zi * ZJ + zj];
+        R totm = R(0), depn = R(0), chtemp = R(0);
+        double sb_out = 0;
+        for (int k = 0; k < S; k++) {
+            const R* sk = s_sed + k * EXT;
+            const R sc = sk[e];
+            const R fkxl = fx0l > R(0) ? fx0l * sk[exl] : fx0l * sc;
+            const R fkxr = fx0r > R(0) ? fx0r * sc : fx0r * sk[exr];
+            const R fkyl = fy0l > R(0) ? fy0l * sk[eyl] : fy0l * sc;
+            const R fkyr = fy0r > R(0) ? fy0r * sc : fy0r * sk[eyr];
+            if (gi == 0) sb_out -= (double)fkxl;
+            if (gi == nx - 1) sb_out += (double)fkxr;
+            if (gj == 0) sb_out -= (double)fkyl;
+            if (gj == ny - 1) sb_out += (double)fkyr;
+            R u = chin[k * plane + c] - dx_ * (fkxr - fkxl) - dx_ * (fkyr - fkyl);
+            R m = A.M[k * plane + c];
+            // sources of class k from the state before this update (SWEHR.c:1322-1399)
+            const R dk = R(A.ds[k]), pk = R(A.pf[k]);
+            R e1 = R(0), e2 = R(0), e3 = R(0), e4 = R(0);
+            if (active) {
+                e1 = (R(1) - H) * pk * R(A.a) * Rr;
+                if (totm_old > R(1e-8) && m > R(1e-8)) e2 = H * m / totm_old * R(A.ad) * Rr;
+                e1 = e1 * shield;
+                e2 = e2 * shield;
+                const R sp = (omega > R(0) && h > dk) ? omega : R(0);
+                e3 = (R(1) - H) * pk * R(A.F) / R(A.J) * sp;
+                if (totm_old > R(1e-6) && m > R(0))
+                    e4 = (R(1) - R(kBetaBedload)) * H * m / totm_old * R(A.F) / e4den * sp;
+            }
+            const R ak = R(13.95) * R(kNu) / dk;
+            R vf = sqrt(ak * ak + cg * dk) - ak;
+            vf = vf * hind;
+            if (dk > R(0.0001) && shallow_v) vf = R(2e-3) + vshal * (vf - R(2e-3)) / (R(6) * h0);
+            if (dk > R(0.002) && shallow_v) vf = R(2e-3) + vshal * (vf - R(2e-3)) / (R(6) * h0);
+            if (shutoff) { e1 = e2 = e3 = e4 = R(0); vf = R(0); }
+            // exchange with the deposited layer
+            if (hn < R(10) * hfilm) {
+                m += dt * e1 + u;
+                bud[B_DET] += (double)(dt * e1);
+                u = R(0);
+            } else {
+                chtemp = u;
+                const R red = fmin(m, dt * (e2 + e4));
+                const R decay = exp(-vf / hn * (R(0.5) * dt));
+                u = decay * u;
+                u = u + dt * (e1 + e3) + red;
+                u = decay * u;
+                zn_c = zn_c - kbed * (u - chtemp);
+                m = m + dt * (e1 + e3) - (u - chtemp);
+                bud[B_DET] += (double)(dt * (e1 + e3));
+                if (fail) {
+                    zn_c = zn_c - R(0.99) * m / rsp;
+                    u = u + R(0.99) * m;
+                    totm = R(0);
+                    m = R(0.01) * m;
+                    depn = R(0);
+                }
+            }
+            const R dh = (u - chtemp) / rsp;
+            hn = hn + dh;
+            bud[B_EXCH] += (double)dh;
+            totm += m;
+            depn += m / rhos;
+            A.ch[q][k * plane + c] = u;
+            A.M[k * plane + c] = m;
+        }
+
+        // dry-cell clamp (SWEHR.c:1532-1551)
+        if (!(hn > hfilm)) {
+            bud[B_CLAMP] += (double)(hfilm - hn);
+            hn = hfilm;
+            hun = R(0);
+            hvn = R(0);
+            for (int k = 0; k < S; k++) {
+                const R u = A.ch[q][k * plane + c];
+                zn_c = zn_c + kbed * u;
+                A.M[k * plane + c] = A.M[k * plane + c] + u;
+                A.ch[q][k * plane + c] = R(0);
+            }
+        }
+        // implicit friction (SWEHR.c:1553-1565) and maxima (SWEHR.c:1587-1588)
+        R v1 = R(0), v2 = R(0);
+        if (hn > hfilm) {
+            const R mn = (hn <= hc) ? rough * R(pow(hn / hc, R(kDepthExp))) : rough;
+            const R sf2 = mn * mn * gz * sqrt(hun * hun + hvn * hvn) / R(pow(hn, R(2.333)));
+            hun = hun / (R(1) + dt * sf2);
+            hvn = hvn / (R(1) + dt * sf2);
+            v1 = hun / hn;
+            v2 = hvn / hn;
+        }
+        const R speed = sqrt(v1 * v1 + v2 * v2);
+        if (speed > A.maxvel[c]) A.maxvel[c] = speed;
+        if (hn > A.maxdepth[c]) A.maxdepth[c] = hn;
+        A.h[q][c] = hn;
+        A.hu[q][c] = hun;
+        A.hv[q][c] = hvn;
+        zn[c] = zn_c;
+        A.totm[c] = totm;
+        A.dep[q][c] = depn;
+
+        // water and sediment leaving through the domain edges this step
+        double wb = 0;
+        if (gi == 0) wb -= (double)fx0l;
+        if (gi == nx - 1) wb += (double)fx0r;
+        if (gj == 0) wb -= (double)fy0l;
+        if (gj == ny - 1) wb += (double)fy0r;
+        bud[B_BWATER] = wb * dtd * A.dx;
+        bud[B_BSED] = sb_out * dtd * A.dx;
+        const double a2 = A.dx * A.dx;
+        bud[B_EXCH] *= a2;
+        bud[B_CLAMP] *= a2;
+        bud[B_DET] *= a2;
+    }
+
+    // ---- per-block budget partials, accumulated across steps in the block's own slot
+    const int lane = tid & 63, wid = tid >> 6;
+#pragma unroll
+    for (int k = 0; k < NB_TERMS; k++) {
+        const double v = wave_sum(bud[k]);
+        if (lane == 0) s_red[k * 16 + wid] = v;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double *slot = A.bud + (size_t)tile * NB_TERMS * 2;
+        for (int k = 0; k < NB_TERMS; k++) {
+            double v = 0;
+            for (int w = 0; w < NT / 64; w++) v += s_red[k * 16 + w];
+            double s = slot[2 * k], cc = slot[2 * k + 1];
+            neumaier(s, cc, v);
+            slot[2 * k] = s;
+            slot[2 * k + 1] = cc;
+        }
+    }
+}
+
+// --------------------------------------------------------------------------- K3
+
+// Derived quantities of a stored cell (SWEHR.c:1516-1531, 1563-1564).
+template <typename R>
+__device__ __forceinline__ void derived(const StepArgs<R> &A, int p, long long c, double &h, double &v1, double &v2,
+                                        double &ts)
+{
+    const R hh = A.h[p][c];
+    h = (double)hh;
+    v1 = v2 = ts = 0;
+    if (hh > R(kHfilm)) {
+        v1 = (double)(A.hu[p][c] / hh);
+        v2 = (double)(A.hv[p][c] / hh);
+        R s = R(0);
+        for (int k = 0; k < A.S; k++) s += A.ch[p][k * A.plane + c] / hh / R(kRhos);
+        ts = (double)s;
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void control_kernel(StepArgs<R> A)
+{
+    DevCtrl *C = A.ctrl;
+    if (!step_active(C)) return;
+    __shared__ double s_best[256];
+    __shared__ int s_idx[256];
+    const long long steps = C->steps;
+    const double t = C->t;
+    const double smax = __longlong_as_double((long long)C->smax_bits);
+    const double dt = step_dt(smax, A.cstable, A.dx, A.maxstep, t, A.tend);
+    const double courant = (smax)*dt * (1.0 / A.dx);
+    const double Rr = rain_rate(A.rain, A.rnum, A.rint, t);
+    const bool cfl = courant > 1;
+    const double tnew = (cfl ? 2 * A.tend : t) + dt;
+    const double stageinterval = C->stageinterval;
+    const int q = (int)((steps + 1) & 1);
+    const int cap = A.stage_cap;
+    __syncthreads();
+    if (tnew > stageinterval) {
+        // outlet: deepest cell of row j = 1, first one on ties (SWEHR.c:1596-1604)
+        double best = 0;
+        int idx = 0;
+        for (int i = threadIdx.x; i < A.nx; i += blockDim.x) {
+            const double hv = (double)A.h[q][(long long)i * A.py];
+            if (hv > best) { best = hv; idx = i; }
+        }
+        s_best[threadIdx.x] = best;
+        s_idx[threadIdx.x] = best > 0 ? idx : 0x7fffffff;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double bb = 0;
+            int bi = 0x7fffffff;
+            for (int k = 0; k < (int)blockDim.x; k++)
+                if (s_best[k] > bb || (s_best[k] == bb && bb > 0 && s_idx[k] < bi)) { bb = s_best[k]; bi = s_idx[k]; }
+            const int outlet = bb > 0 ? bi : 0;
+            const int r = C->stagecnt;
+            if (r >= cap) {
+                C->status = 6;  // SWEHR_ERR_STAGE_FULL
+            } else {
+                const long long c = (long long)outlet * A.py;
+                double h, v1, v2, ts;
+                derived(A, q, c, h, v1, v2, ts);
+                double *st = A.stage;
+                st[0 * cap + r] = tnew;
+                st[1 * cap + r] = h;
+                st[2 * cap + r] = sqrt(v1 * v1 + v2 * v2);
+                st[3 * cap + r] = ts;
+                for (int k = 0; k < 10; k++) {
+                    st[(4 + k) * cap + r] = k < A.S ? (double)A.M[k * A.plane + c] : 0.0;
+                    st[(14 + k) * cap + r] = k < A.S ? (double)A.ch[q][k * A.plane + c] : 0.0;
+                }
+                const int yp[6] = {1, 10, 20, 30, 40, 50};
+                for (int n = 0; n < 6; n++) {
+                    double d = 0, sp = 0, tt = 0;
+                    if (A.nx >= 2 && yp[n] <= A.ny) {
+                        double hh, w1, w2;
+                        derived(A, q, (long long)1 * A.py + (yp[n] - 1), hh, w1, w2, tt);
+                        d = hh;
+                        sp = sqrt(w1 * w1 + w2 * w2);
+                    }
+                    A.saveflow[(3 * n + 0) * cap + r] = d;
+                    A.saveflow[(3 * n + 1) * cap + r] = sp;
+                    A.saveflow[(3 * n + 2) * cap + r] = tt;
+                }
+                C->stagecnt = r + 1;
+            }
+            C->stageinterval = stageinterval + 1;
+        }
+    }
+    if (threadIdx.x == 0) {
+        if (tnew >= C->printinterval) {
+            C->printinterval = C->printinterval + C->printint;
+            if (C->pause_on_print) C->print_due = 1;
+        }
+        if (cfl) C->status = 1;  // SWEHR_ERR_CFL
+        // dt = 0 (infinite wave speed) would never reach tend; the reference loops forever
+        if (!(dt > 0) && tnew < A.tend) C->status = 2;  // SWEHR_ERR_NONFINITE
+        double rs = C->rain_sum, rc = C->rain_comp;
+        neumaier(rs, rc, dt * Rr * ((double)A.nx * A.ny) * A.dx * A.dx);
+        C->rain_sum = rs;
+        C->rain_comp = rc;
+        C->t = tnew;
+        C->dt = dt;
+        C->courant = courant;
+        C->rain = Rr;
+        C->steps = steps + 1;
+        C->smax_bits = 0ull;
+        if (!(tnew < A.tend) || C->status != 0) C->done = 1;
+    }
+}
+
+}  // namespace swehr

@@ -1,0 +1,215 @@
+"""Python access to the HIP solver through its C ABI (include/swehr.h).
+
+The library ``swehr_amd/libswehr_hip.so`` is built in-tree by ``__graft_entry__.build()``
+(``make -C swehr_amd/csrc``).  There is no fallback: if the library is missing, or no
+GPU is present, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+from . import _abi
+from ._abi import dptr
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB_PATH = os.path.join(HERE, "libswehr_hip.so")
+HEADER = os.path.join(ROOT, "include", "swehr.h")
+DRIVER = os.path.join(HERE, "bin", "swehr")
+
+_lib = None
+
+
+class SwehrError(RuntimeError):
+    def __init__(self, where: str, code: int, msg: str = ""):
+        self.code = code
+        super().__init__("%s: %s%s" % (where, _abi.STATUS_NAMES.get(code, str(code)), (": " + msg) if msg else ""))
+
+
+def header_functions(path: str = HEADER):
+    """Names of the functions include/swehr.h declares."""
+    with open(path) as fh:
+        text = fh.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s+\**\s*(swehr_[a-z_0-9]+)\s*\(", text, flags=re.M)))
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("%s is not built: run __graft_entry__.build() (make -C swehr_amd/csrc)" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER(C.c_double)
+    L.swehr_params_init.argtypes = [C.POINTER(_abi.Params)]
+    L.swehr_create.argtypes = [C.POINTER(_abi.Params), P, P, P, P, P, C.c_int, C.POINTER(C.c_void_p)]
+    L.swehr_advance.argtypes = [C.c_void_p, C.c_int64, C.POINTER(_abi.StepInfo)]
+    L.swehr_ack_print.argtypes = [C.c_void_p]
+    L.swehr_download.argtypes = [C.c_void_p, C.c_int, C.c_int, P]
+    L.swehr_stage.argtypes = [C.c_void_p, P, P, C.c_int, C.POINTER(C.c_int)]
+    L.swehr_budget.argtypes = [C.c_void_p, P]
+    L.swehr_instance_bytes.restype = C.c_int64
+    L.swehr_instance_bytes.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
+    L.swehr_destroy.argtypes = [C.c_void_p]
+    L.swehr_strerror.restype = C.c_char_p
+    L.swehr_strerror.argtypes = [C.c_int]
+    L.swehr_last_error.restype = C.c_char_p
+    L.swehr_configure.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+    L.swehr_tile_shape.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.swehr_get_stream.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
+    L.swehr_profile.argtypes = [C.c_void_p, C.c_int, P, C.POINTER(C.c_int)]
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    v = lib().swehr_last_error()
+    return v.decode() if v else ""
+
+
+def check(where: str, rc: int, allow=(0,)) -> int:
+    if rc not in allow:
+        raise SwehrError(where, rc, last_error())
+    return rc
+
+
+def default_params() -> _abi.Params:
+    p = _abi.Params()
+    lib().swehr_params_init(C.byref(p))
+    return p
+
+
+def instance_bytes(nx: int, ny: int, nclass: int, precision: int = _abi.FP64) -> int:
+    return int(lib().swehr_instance_bytes(nx, ny, nclass, precision))
+
+
+def tile_shapes():
+    L = lib()
+    out = []
+    for k in range(1, L.swehr_tile_count() + 1):
+        a, b = C.c_int(), C.c_int()
+        L.swehr_tile_shape(k, C.byref(a), C.byref(b))
+        out.append((a.value, b.value))
+    return out
+
+
+class Solver:
+    """One GPU instance of a ``swehr_amd.inputs.Case``."""
+
+    def __init__(self, case, precision: int = _abi.FP64, pause_on_print: bool = False, steps_per_graph: int = 0,
+                 tile: int = 0):
+        L = lib()
+        self.case = case
+        self.precision = precision
+        self._p = _abi.params_from_case(case)
+        topo, ks, ds, pf, rain = _abi.case_arrays(case)
+        h = C.c_void_p()
+        rc = L.swehr_create(C.byref(self._p), dptr(topo), dptr(ks) if ks is not None else None, dptr(ds), dptr(pf),
+                            dptr(rain), int(precision), C.byref(h))
+        check("swehr_create", rc)
+        self._h = h
+        check("swehr_configure", L.swehr_configure(self._h, 1 if pause_on_print else 0, int(steps_per_graph),
+                                                   int(tile)))
+
+    def configure(self, pause_on_print: int = -1, steps_per_graph: int = 0, tile: int = 0):
+        check("swehr_configure", lib().swehr_configure(self._h, int(pause_on_print), int(steps_per_graph), int(tile)))
+
+    def advance(self, n: int = 1 << 62) -> dict:
+        info = _abi.StepInfo()
+        rc = lib().swehr_advance(self._h, int(n), C.byref(info))
+        check("swehr_advance", rc, allow=(_abi.OK, _abi.ERR_CFL))
+        return info.as_dict()
+
+    def run(self, n: int = 1 << 62) -> dict:
+        """Advance n steps (or to the end), acknowledging print pauses."""
+        left = int(n)
+        info = self.info()
+        while left > 0 and not info["done"]:
+            before = info["steps"]
+            info = self.advance(left)
+            left -= info["steps"] - before
+            if info["print_due"]:
+                self.ack_print()
+        return info
+
+    def ack_print(self):
+        check("swehr_ack_print", lib().swehr_ack_print(self._h))
+
+    def info(self) -> dict:
+        return self.advance(0)
+
+    def field(self, name: str, k: int = 0) -> np.ndarray:
+        out = np.empty(self.case.nx * self.case.ny, dtype=np.float64)
+        check("swehr_download", lib().swehr_download(self._h, _abi.FIELDS[name], int(k), dptr(out)))
+        return out.reshape(self.case.nx, self.case.ny)
+
+    def state(self) -> dict:
+        S = self.case.nclass
+        d = {k: self.field(k) for k in ("depth", "uh", "vh", "topo", "vinf", "totm", "dep", "totsed", "maxvel",
+                                        "maxdepth", "velx", "vely")}
+        d["ch"] = np.stack([self.field("ch", k + 1) for k in range(S)])
+        d["m"] = np.stack([self.field("m", k + 1) for k in range(S)])
+        return d
+
+    def stage(self):
+        L = lib()
+        cnt = C.c_int(0)
+        check("swehr_stage", L.swehr_stage(self._h, None, None, 0, C.byref(cnt)))
+        cap = max(1, cnt.value)
+        st = np.zeros(_abi.STAGE_COLS * cap)
+        fl = np.zeros(_abi.FLOW_COLS * cap)
+        check("swehr_stage", L.swehr_stage(self._h, dptr(st), dptr(fl), cap, C.byref(cnt)))
+        n = cnt.value
+        return st.reshape(_abi.STAGE_COLS, cap)[:, :n].copy(), fl.reshape(_abi.FLOW_COLS, cap)[:, :n].copy()
+
+    def budget(self) -> np.ndarray:
+        out = np.zeros(_abi.BUDGET_N)
+        check("swehr_budget", lib().swehr_budget(self._h, dptr(out)))
+        return out
+
+    def profile(self, nsteps: int):
+        """Average kernel durations (ms) of wave-speed, step and control kernels over up
+        to nsteps eagerly launched steps; returns (avg_ms[3], steps_taken)."""
+        out = np.zeros(3)
+        n = C.c_int(0)
+        check("swehr_profile", lib().swehr_profile(self._h, int(nsteps), dptr(out), C.byref(n)))
+        return out, n.value
+
+    def stream(self) -> int:
+        s = C.c_void_p()
+        check("swehr_get_stream", lib().swehr_get_stream(self._h, C.byref(s)))
+        return int(s.value or 0)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().swehr_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def water_residual(b: np.ndarray) -> float:
+    """Water budget residual (m^3): storage change minus the sum of all sources."""
+    stored = b[0] + b[1]
+    sources = b[2] - b[3] + b[9] + b[10]
+    return float(stored - sources)
+
+
+def sediment_residual(b: np.ndarray) -> float:
+    """Sediment budget residual (kg): suspended + deposited minus detachment and inflow."""
+    return float(b[4] + b[5] - (b[8] - b[7]))

@@ -1,0 +1,119 @@
+"""HIP path against the CPU oracle on the same inputs (needs an MI355X).
+
+The FP64 kernel keeps the reference's operation order but replaces some libm calls by
+identities that are exact in real arithmetic (sqrt for pow(x, .5), 1/sqrt(1+x^2) for
+cos(atan x), one exp instead of two equal ones) and uses the GPU's own exp/pow, so it
+agrees with the oracle to rounding level, not bit for bit.  Over a few hundred steps
+the relative L2 error of every field stays below RTOL_FP64_SHORT; whole runs of the
+golden cases stay below RTOL_FP64_RUN.  Time and step counts are compared as well.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from golden.cases import BY_NAME
+from oracle_lib import Oracle
+from parity_util import compare_states, rel_l2, worst
+from swehr_amd import _abi
+from swehr_amd import solver as sv
+from swehr_amd.inputs import read_case
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+
+RTOL_FP64_SHORT = 1e-10
+RTOL_FP64_RUN = 1e-7
+
+
+def _case(name):
+    return read_case(os.path.join(GOLD, name, "inputs"), name=name)
+
+
+def _pair(case, steps, precision=_abi.FP64):
+    o = Oracle(case)
+    o.run(steps)
+    s = sv.Solver(case, precision=precision)
+    s.run(steps)
+    return o, s
+
+
+SHORT = [("shipped", 1), ("shipped", 40), ("inflow_mix", 1), ("inflow_mix", 300), ("narrow", 200),
+         ("rain_plane", 120), ("transmissive", 150), ("closed_basin", 150), ("cfl_abort", 60)]
+
+
+@pytest.mark.parametrize("name,steps", SHORT, ids=["%s-%d" % x for x in SHORT])
+def test_fp64_steps_match_oracle(name, steps):
+    case = _case(name)
+    print("case", name, "steps", steps, flush=True)
+    o, s = _pair(case, steps)
+    oi, si = o.info(), s.info()
+    assert si["steps"] == oi["steps"]
+    assert si["t"] == pytest.approx(oi["t"], rel=1e-12, abs=1e-12)
+    errs = compare_states(s.state(), o.state(), case.nclass)
+    print({k: "%.2e" % v for k, v in errs.items()})
+    k, e = worst(errs)
+    assert e <= RTOL_FP64_SHORT, (k, e)
+    st_g, fl_g = s.stage()
+    st_o, fl_o = o.stage()
+    assert st_g.shape == st_o.shape
+    assert rel_l2(st_g, st_o) <= RTOL_FP64_SHORT
+    assert rel_l2(fl_g, fl_o) <= RTOL_FP64_SHORT
+    s.close()
+
+
+@pytest.mark.parametrize("name", ["inflow_mix", "narrow", "rain_plane", "closed_basin", "cfl_abort"])
+def test_fp64_whole_run_matches_oracle(name):
+    case = _case(name)
+    print("case", name, "whole run", flush=True)
+    o, s = _pair(case, 1 << 40)
+    oi, si = o.info(), s.info()
+    assert si["done"] == 1 and oi["done"] == 1
+    assert si["status"] == oi["status"]
+    assert abs(si["steps"] - oi["steps"]) <= max(2, oi["steps"] // 1000)
+    errs = compare_states(s.state(), o.state(), case.nclass)
+    print({k: "%.2e" % v for k, v in errs.items()})
+    k, e = worst(errs)
+    assert e <= RTOL_FP64_RUN, (k, e)
+    s.close()
+
+
+@pytest.mark.parametrize("name", ["transmissive", "closed_basin"])
+def test_fp64_mass_budgets_close(name):
+    """Water and sediment budgets close to rounding, and every budget term agrees with
+    the oracle's (terms are compared relative to the largest water / sediment term)."""
+    case = _case(name)
+    with sv.Solver(case) as s:
+        s.run()
+        b = s.budget()
+    o = Oracle(case)
+    o.run()
+    bo = o.budget()
+    wscale = max(abs(bo[n]) for n in (0, 1, 2, 3, 9, 10))
+    sscale = max(max(abs(bo[n]) for n in (4, 5, 7, 8)), 1e-30)
+    print("gpu", b, "\noracle", bo)
+    assert abs(sv.water_residual(b)) <= 1e-11 * wscale
+    assert abs(sv.sediment_residual(b)) <= 1e-11 * sscale
+    for n in (0, 1, 2, 3, 9, 10):
+        assert abs(b[n] - bo[n]) <= 1e-7 * wscale, n
+    for n in (4, 5, 7, 8):
+        assert abs(b[n] - bo[n]) <= 1e-7 * sscale, n
+    assert abs(b[6] - bo[6]) <= 1e-7 * max(abs(bo[6]), 1e-9)
+    assert b[11] == bo[11] or abs(b[11] - bo[11]) <= max(2, bo[11] // 1000)
+
+
+@pytest.mark.parametrize("name", ["transmissive", "closed_basin", "rain_plane"])
+def test_fp32_form_close_to_fp64_oracle(name):
+    """Precision form: FP32 state against the FP64 oracle over a whole run; depth within
+    1e-4 relative L2 and the water budget closed to 1e-6 of the water moved."""
+    case = _case(name)
+    o, s = _pair(case, 1 << 40, precision=_abi.FP32)
+    errs = compare_states(s.state(), o.state(), case.nclass)
+    print({k: "%.2e" % v for k, v in errs.items()})
+    assert errs["depth"] <= 1e-4
+    b = s.budget()
+    wscale = max(abs(b[n]) for n in (0, 1, 2, 3, 9, 10))
+    assert abs(sv.water_residual(b)) <= 1e-6 * wscale
+    s.close()
