@@ -190,6 +190,63 @@ int swehr_get_stream(swehr_handle *h, void **stream);
  * kernels in avg_ms[0..2]; *taken receives the number of steps run. */
 int swehr_profile(swehr_handle *h, int nsteps, double *avg_ms, int *taken);
 
+/* Checkpoint of the instance's whole device state (snapshot/restore): *bytes is the
+ * snapshot size; swehr_snapshot copies the state into host memory of that size, and
+ * swehr_restore loads a snapshot taken from an instance created with the same
+ * parameters.  No counterpart in the reference (its final text rasters are lossy). */
+int swehr_snapshot_bytes(swehr_handle *h, int64_t *bytes);
+int swehr_snapshot(swehr_handle *h, void *host);
+int swehr_restore(swehr_handle *h, const void *host);
+
+/* Tile-shape autotuner: time the step kernel for nsteps steps with every tile shape
+ * that fits (state is saved on the device first and restored after each trial),
+ * select the fastest and return it in *best; ms_per_tile (swehr_tile_count()
+ * entries, may be NULL) receives the average kernel time of each shape (0 = unfit). */
+int swehr_autotune(swehr_handle *h, int nsteps, int *best, double *ms_per_tile);
+
+/* ---- ensembles: many instances (different sizes and parameters, one precision)
+ * stepped together, each with its own dt; every stage of a step is one launch over
+ * the tiles of all unfinished instances, and steps run as captured HIP graphs. */
+typedef struct swehr_batch swehr_batch;
+
+/* n instances; per-instance arguments as for swehr_create (ks[i] may be NULL).
+ * tile: tile shape id for all instances (0 = automatic). */
+int swehr_batch_create(int n, const swehr_params *params, const double *const *topo, const double *const *ks,
+                       const double *const *ds, const double *const *pfrac, const double *const *rain,
+                       int precision, int tile, swehr_batch **out);
+/* Up to max_steps steps of every unfinished instance; *remaining receives the number
+ * of instances not yet done. */
+int swehr_batch_advance(swehr_batch *b, int64_t max_steps, int *remaining);
+/* Instance i as a handle (swehr_download, swehr_stage, swehr_budget, and
+ * swehr_advance with max_steps = 0 for its state). Owned by the batch. */
+int swehr_batch_handle(swehr_batch *b, int i, swehr_handle **h);
+/* Autotune the shared tile shape over the whole ensemble (as swehr_autotune). */
+int swehr_batch_autotune(swehr_batch *b, int nsteps, int *best, double *ms_per_tile);
+int swehr_batch_destroy(swehr_batch *b);
+
+/* ---- serving: continuous batching of solve requests.  Requests wait in a queue
+ * and are admitted while an instance slot is free and the server's device-memory
+ * budget holds their state; admitted requests step together (ensemble launches) and
+ * stay resident in HBM until released. */
+typedef struct swehr_server swehr_server;
+
+int swehr_server_create(int64_t mem_bytes, int max_slots, int precision, int tile, swehr_server **out);
+/* Queue a request (inputs are copied); *id identifies it.  SWEHR_ERR_BUSY if its
+ * state can never fit the memory budget. */
+int swehr_server_submit(swehr_server *s, const swehr_params *p, const double *topo, const double *ks,
+                        const double *ds, const double *pfrac, const double *rain, int64_t *id);
+/* Admit what fits, take up to steps time steps of every running request, retire the
+ * finished ones.  Counts after the call go to *running, *queued, *finished. */
+int swehr_server_pump(swehr_server *s, int steps, int *running, int *queued, int *finished);
+/* Pop one finished request not yet reported: its id and final step info;
+ * SWEHR_ERR_BUSY when there is none. */
+int swehr_server_next_done(swehr_server *s, int64_t *id, swehr_step_info *info);
+/* The finished request's instance (fields, stage series, budgets); owned by the server. */
+int swehr_server_request(swehr_server *s, int64_t id, swehr_handle **h);
+/* Free a finished request's device memory. */
+int swehr_server_release(swehr_server *s, int64_t id);
+int swehr_server_destroy(swehr_server *s);
+
 /* Last error message of the calling thread (empty when none). */
 const char *swehr_last_error(void);
 

@@ -1,6 +1,6 @@
-// swehr_device.h - device-side data structures and the three per-step kernels of the
-// SWEHR coupled time step on gfx950.  Shared by the single-instance library
-// (swehr_lib.hip) and the batched/serving runners (swehr_batch.hip).
+// swehr_device.h - device-side data structures and the per-step kernels of the SWEHR
+// coupled time step on gfx950.  Shared by the single-instance library
+// (swehr_lib.hip) and the ensemble / serving runner (swehr_batch.hip).
 //
 // Memory layout in HBM (one instance): structure of arrays, one plane per variable,
 // value (i, j) at [i * py + j] with j (downslope) unit-stride and the row pitch py
@@ -13,14 +13,17 @@
 // are not stored: they are exact functions of (h, ch_k, hu, hv) (SWEHR.c:1516-1531,
 // 1563-1564, 1578-1579) and are recomputed where needed.
 //
-// One step = three launches, all guarded by the device control block so that a
-// captured graph of many steps stops by itself at t >= tend, at a print pause, or at
-// a step limit:
-//   K1 wave_speed_kernel  max |wave speed| over every face (SWEHR.c:1161, 1228)
-//                         -> atomicMax on the bit pattern of a non-negative double
-//   K2 step_kernel        boundary ghosts, fluxes, sources, dt, update (SWEHR.c:816-1589)
-//                         on LDS-staged tiles; every face flux computed once per tile
-//   K3 control_kernel     t += dt, outlet and probe records, print pause (SWEHR.c:1591-1682)
+// One step = three launches, all guarded by the per-instance device control block so
+// that a captured graph of many steps stops by itself at t >= tend, at a print pause
+// or at a step limit:
+//   K1 wave speeds  max |wave speed| over every face (SWEHR.c:1161, 1228) -> atomicMax
+//                   on the bit pattern of a non-negative double
+//   K2 step         boundary ghosts, fluxes, sources, dt, update (SWEHR.c:816-1589) on
+//                   LDS-staged tiles; every face flux is computed once per tile and the
+//                   per-class work streams through registers
+//   K3 control      t += dt, outlet and probe records, print pause (SWEHR.c:1591-1682)
+// The bodies are device functions; the single-instance kernels and the ensemble
+// kernels (one launch over the tiles of many instances) call the same code.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -34,7 +37,7 @@ constexpr int kStageCols = 24;
 constexpr int kFlowCols = 18;
 constexpr int kStageMax = 30000;
 
-// per-block budget terms accumulated across steps (sum and Neumaier compensation)
+// per-tile budget terms accumulated across steps (sum and Neumaier compensation)
 enum BudgetTerm { B_EXCH = 0, B_CLAMP = 1, B_DET = 2, B_BWATER = 3, B_BSED = 4, NB_TERMS = 5 };
 
 struct alignas(64) DevCtrl {
@@ -60,11 +63,13 @@ struct StepArgs {
     double ks, dx, h0, mtstar0, a, ad, F, J, roughness, theta0, thetas, hf, cstable, maxstep, tend, rint, vinf0;
     int rnum, inflow, bndry[4];
     int tiles_i, tiles_j, nblocks, blocks_per_xcd;
-    int bud_slots;   // budget slots allocated (>= nblocks of every tile shape)
-    int stage_cap;   // rows of the stage / saveflow series
+    int bud_slots;  // budget slots allocated (>= tile count of every tile shape)
+    int stage_cap;  // rows of the stage / saveflow series
     double ds[kMaxClasses], pf[kMaxClasses];
+    // per-class settling constants a_k = 13.95 nu / d_k and a_k^2 (SWEHR.c:435)
+    double ak[kMaxClasses], ak2[kMaxClasses];
     DevCtrl *ctrl;
-    double *bud;       // [nblocks][NB_TERMS][2]
+    double *bud;       // [bud_slots][NB_TERMS][2]
     double *stage;     // [kStageCols][stage_cap]
     double *saveflow;  // [kFlowCols][stage_cap]
 };
@@ -116,19 +121,17 @@ __device__ __forceinline__ R slope_y(const R *P, int zi, int zj, int gj, int ny,
     return odx * (P[zi * ZJ + zj + 1] - P[zi * ZJ + zj]);
 }
 
-// cos(atan|x|) = 1 / sqrt(1 + x^2)
-template <typename R>
-__device__ __forceinline__ R cos_atan(R x)
-{
-    return R(1) / sqrt(R(1) + x * x);
-}
-
 // --------------------------------------------------------------------------- K1
 
-// Global-memory slope for the wave-speed pass (no LDS): z at clamped neighbours.
+// Wave-speed bound of the faces a cell owns: i+1/2 and j+1/2, plus 1/2 at i = 0 or
+// j = 0.  Ghost states follow the boundary rules of SWEHR.c:816-1035; GZ is that of
+// the previous step's flux pass, from the lagged bed zp (read from global memory).
 template <typename R>
-__device__ __forceinline__ R gz_from_plane(const R *zp, int i, int j, int nx, int ny, int py, R odx)
+__device__ __forceinline__ R wave_speed_cell(const StepArgs<R> &A, const R *h, const R *hu, const R *hv,
+                                             const R *zp, int i, int j, const InflowState &in)
 {
+    const int nx = A.nx, ny = A.ny, py = A.py;
+    const R odx = R(1.0 / A.dx);
     const long long c = (long long)i * py + j;
     R sx, sy;
     if (i != 0 && i != nx - 1) sx = R(0.5) * odx * (zp[c + py] - zp[c - py]);
@@ -137,7 +140,47 @@ __device__ __forceinline__ R gz_from_plane(const R *zp, int i, int j, int nx, in
     if (j != 0 && j != ny - 1) sy = R(0.5) * odx * (zp[c + 1] - zp[c - 1]);
     else if (j == ny - 1) sy = odx * (zp[c] - zp[j > 0 ? c - 1 : c]);
     else sy = odx * (zp[ny > 1 ? c + 1 : c] - zp[c]);
-    return R(kG) / sqrt(sx * sx + sy * sy + R(1));
+    const R g = R(kG) * rsqrt(sx * sx + sy * sy + R(1));
+    const R hc = h[c], huc = hu[c], hvc = hv[c];
+    R smax = R(0), s;
+    {
+        R hr, qr;
+        if (i < nx - 1) { hr = h[c + py]; qr = hu[c + py]; }
+        else { hr = hc; qr = A.bndry[2] == 1 ? -huc : huc; }
+        s = face_speed(hc, hr, huc, qr, g);
+        smax = s > smax ? s : smax;
+    }
+    {
+        R hr, qr;
+        if (j < ny - 1) { hr = h[c + 1]; qr = hv[c + 1]; }
+        else if (A.inflow) { hr = R(in.h); qr = R(in.hv); }
+        else { hr = hc; qr = A.bndry[3] == 1 ? -hvc : hvc; }
+        s = face_speed(hc, hr, hvc, qr, g);
+        smax = s > smax ? s : smax;
+    }
+    if (i == 0) {  // face 1/2: ghost on the left, GZ of cell 1 (SWEHR.c:1054-1059)
+        s = face_speed(hc, hc, A.bndry[0] == 1 ? -huc : huc, huc, g);
+        smax = s > smax ? s : smax;
+    }
+    if (j == 0) {
+        s = face_speed(hc, hc, A.bndry[1] == 1 ? -hvc : hvc, hvc, g);
+        smax = s > smax ? s : smax;
+    }
+    return smax;
+}
+
+// Block-wide max of v, then one atomicMax on the instance's Smax.
+__device__ __forceinline__ void block_max_to(unsigned long long *dst, double v)
+{
+    __shared__ double red[16];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    v = wave_max(v);
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)((blockDim.x + 63) >> 6); w++) v = red[w] > v ? red[w] : v;
+        atomicMax(dst, (unsigned long long)__double_as_longlong(v));
+    }
 }
 
 template <typename R>
@@ -147,57 +190,33 @@ __global__ __launch_bounds__(256) void wave_speed_kernel(StepArgs<R> A)
     if (!step_active(C)) return;
     const long long steps = C->steps;
     const int p = (int)(steps & 1), p3 = (int)(steps % 3);
-    const R *h = A.h[p], *hu = A.hu[p], *hv = A.hv[p];
-    const R *zp = A.z[(p3 + 2) % 3];
-    const int nx = A.nx, ny = A.ny, py = A.py;
-    const R odx = R(1.0 / A.dx);
-    const double t = C->t;
-    InflowState in = inflow_state(t);
+    const InflowState in = inflow_state(C->t);
     R smax = R(0);
-    const long long n = (long long)nx * ny;
+    const long long n = (long long)A.nx * A.ny;
     for (long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x; cell < n;
          cell += (long long)gridDim.x * blockDim.x) {
-        const int i = (int)(cell / ny), j = (int)(cell % ny);
-        const long long c = (long long)i * py + j;
-        const R g = gz_from_plane(zp, i, j, nx, ny, py, odx);
-        const R hc = h[c], huc = hu[c], hvc = hv[c];
-        // face i+1/2 (right ghost at i = nx-1)
-        {
-            R hr, qr;
-            if (i < nx - 1) { hr = h[c + py]; qr = hu[c + py]; }
-            else { hr = hc; qr = A.bndry[2] == 1 ? -huc : huc; }
-            const R s = face_speed(hc, hr, huc, qr, g);
-            smax = s > smax ? s : smax;
-        }
-        // face j+1/2 (top ghost at j = ny-1: inflow or side rule)
-        {
-            R hr, qr;
-            if (j < ny - 1) { hr = h[c + 1]; qr = hv[c + 1]; }
-            else if (A.inflow) { hr = R(in.h); qr = R(in.hv); }
-            else { hr = hc; qr = A.bndry[3] == 1 ? -hvc : hvc; }
-            const R s = face_speed(hc, hr, hvc, qr, g);
-            smax = s > smax ? s : smax;
-        }
-        if (i == 0) {  // face 1/2: ghost on the left, GZ of cell 1 (SWEHR.c:1054-1059)
-            const R hl = hc, ql = A.bndry[0] == 1 ? -huc : huc;
-            const R s = face_speed(hl, hc, ql, huc, g);
-            smax = s > smax ? s : smax;
-        }
-        if (j == 0) {
-            const R hl = hc, ql = A.bndry[1] == 1 ? -hvc : hvc;
-            const R s = face_speed(hl, hc, ql, hvc, g);
-            smax = s > smax ? s : smax;
-        }
+        const int i = (int)(cell / A.ny), j = (int)(cell % A.ny);
+        const R s = wave_speed_cell(A, A.h[p], A.hu[p], A.hv[p], A.z[(p3 + 2) % 3], i, j, in);
+        smax = s > smax ? s : smax;
     }
-    double m = wave_max((double)smax);
-    __shared__ double red[16];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) red[wid] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = red[w] > m ? red[w] : m;
-        atomicMax(&C->smax_bits, (unsigned long long)__double_as_longlong(m));
-    }
+    block_max_to(&C->smax_bits, (double)smax);
+}
+
+// Wave speeds of one TI x TJ tile (ensemble path: one block per tile).
+template <typename R, int TI, int TJ>
+__device__ __forceinline__ void wave_speed_tile(const StepArgs<R> &A, int tile)
+{
+    DevCtrl *C = A.ctrl;
+    if (!step_active(C)) return;
+    const long long steps = C->steps;
+    const int p = (int)(steps & 1), p3 = (int)(steps % 3);
+    const InflowState in = inflow_state(C->t);
+    const int ti = tile / A.tiles_j, tj = tile - (tile / A.tiles_j) * A.tiles_j;
+    const int il = threadIdx.x / TJ, jl = threadIdx.x - (threadIdx.x / TJ) * TJ;
+    const int i = ti * TI + il, j = tj * TJ + jl;
+    R s = R(0);
+    if (i < A.nx && j < A.ny) s = wave_speed_cell(A, A.h[p], A.hu[p], A.hv[p], A.z[(p3 + 2) % 3], i, j, in);
+    block_max_to(&C->smax_bits, (double)s);
 }
 
 // --------------------------------------------------------------------------- K2
@@ -214,21 +233,23 @@ struct TileGeom {
     }
 };
 
+// The fused step for one tile: ghosts, face fluxes, sources, update.
+//
+// Arithmetic follows SWEHR.c operation for operation except where a division or a
+// transcendental call is replaced by an identity exact in real arithmetic (a
+// reciprocal computed once and multiplied, 1/sqrt via rsqrt, c* = |d|/4 in the
+// Riemann solver, the class-invariant source factors hoisted out of the class loop);
+// those change results at the rounding level only.
 template <typename R, int TI, int TJ>
-__global__ __launch_bounds__(TI *TJ) void step_kernel(StepArgs<R> A)
+__device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsigned char *smem_raw)
 {
     using T = TileGeom<R, TI, TJ>;
-    constexpr int NT = T::NT, EJ = T::EJ, EXT = T::EXT, ZI = T::ZI, ZJ = T::ZJ, ZEXT = T::ZEXT;
+    constexpr int NT = T::NT, EJ = T::EJ, EXT = T::EXT, ZJ = T::ZJ, ZEXT = T::ZEXT;
     constexpr int NFX = T::NFX, NFY = T::NFY;
     DevCtrl *C = A.ctrl;
     if (!step_active(C)) return;
 
-    // XCD-aware tile order: consecutive block ids land on different XCDs, so give
-    // each XCD a contiguous band of tiles (their halos then share that XCD's L2).
-    const int b = blockIdx.x;
-    const int tile = (b & 7) * A.blocks_per_xcd + (b >> 3);
-    if (tile >= A.nblocks) return;
-    const int ti = tile / A.tiles_j, tj = tile % A.tiles_j;
+    const int ti = tile / A.tiles_j, tj = tile - (tile / A.tiles_j) * A.tiles_j;
     const int i0 = ti * TI, j0 = tj * TJ;
 
     const long long steps = C->steps;
@@ -241,9 +262,7 @@ __global__ __launch_bounds__(TI *TJ) void step_kernel(StepArgs<R> A)
     const long long plane = A.plane;
     const R *hin = A.h[p], *huin = A.hu[p], *hvin = A.hv[p], *chin = A.ch[p], *depin = A.dep[p];
     const R *zc = A.z[p3], *zp = A.z[(p3 + 2) % 3];
-    R *zn = A.z[(p3 + 1) % 3];
 
-    extern __shared__ __align__(16) unsigned char smem_raw[];
     double *s_red = reinterpret_cast<double *>(smem_raw);
     R *s_h = reinterpret_cast<R *>(s_red + NB_TERMS * 16);
     R *s_hu = s_h + EXT, *s_hv = s_hu + EXT, *s_ts = s_hv + EXT, *s_dep = s_ts + EXT;
@@ -253,6 +272,7 @@ __global__ __launch_bounds__(TI *TJ) void step_kernel(StepArgs<R> A)
 
     const R hfilm = R(kHfilm);
     const R rhos = R(kRhos), rhow = R(kRhow);
+    const R inv_rhos = R(1.0 / kRhos);
     const int tid = threadIdx.x;
 
     // ---- phase 1: ghosts and the 1-cell halo of the conserved state (SWEHR.c:816-1035)
@@ -274,19 +294,18 @@ __global__ __launch_bounds__(TI *TJ) void step_kernel(StepArgs<R> A)
             h = R(in.h);
             hu = R(0);
             hv = R(in.hv);
-            const R cc = R(in.c);
-            for (int k = 0; k < S; k++) s_sed[k * EXT + e] = cc / h;
-        } else {
-            const bool wet = h > hfilm;
+            const R sd = R(in.c) / h;
+            for (int k = 0; k < S; k++) s_sed[k * EXT + e] = sd;
+        } else if (h > hfilm) {
+            const R ih = R(1) / h;
             for (int k = 0; k < S; k++) {
-                R sd = R(0);
-                if (wet) {
-                    sd = chin[k * plane + c] / h;
-                    ts += sd / rhos;
-                    if (sd < R(-1e-5)) sd = R(0);
-                }
+                R sd = chin[k * plane + c] * ih;
+                ts += sd * inv_rhos;
+                if (sd < R(-1e-5)) sd = R(0);
                 s_sed[k * EXT + e] = sd;
             }
+        } else {
+            for (int k = 0; k < S; k++) s_sed[k * EXT + e] = R(0);
         }
         s_h[e] = h;
         s_hu[e] = hu;
@@ -313,7 +332,7 @@ __global__ __launch_bounds__(TI *TJ) void step_kernel(StepArgs<R> A)
         const int zi = gi - (i0 - 2), zj = gj - (j0 - 2);
         const R a = slope_x<R, ZJ>(s_zp, zi, zj, gi, nx, odx);
         const R bb = slope_y<R, ZJ>(s_zp, zi, zj, gj, ny, odx);
-        return g / sqrt(a * a + bb * bb + R(1));
+        return g * rsqrt(a * a + bb * bb + R(1));
     };
 
     // ---- phase 2: face fluxes of the tile, each computed once
@@ -364,80 +383,99 @@ __global__ __launch_bounds__(TI *TJ) void step_kernel(StepArgs<R> A)
         const R sxn = slope_x<R, ZJ>(s_z, zi, zj, gi, nx, odx);
         const R syo = slope_y<R, ZJ>(s_zp, zi, zj, gj, ny, odx);
         const R syn = slope_y<R, ZJ>(s_z, zi, zj, gj, ny, odx);
-        const R gx = g * sxn / sqrt(sxn * sxn + syo * syo + R(1));
+        const R gx = g * sxn * rsqrt(sxn * sxn + syo * syo + R(1));
         const R sxs = -gx * h;
-        const R nrm = sqrt(sxn * sxn + syn * syn + R(1));
-        const R gy = g * syn / nrm;
+        const R n2 = sxn * sxn + syn * syn + R(1);
+        const R rn = rsqrt(n2);
+        const R gy = g * syn * rn;
         const R sys = -gy * h;
-        const R gz = g / nrm;
+        const R gz = g * rn;
 
         // mixture density from the previous update (SWEHR.c:1578-1579)
         R rho = rhow * (R(1) - ts) + rhos * ts;
         if (rho > R(0.8) * rhos) rho = R(0.8) * rhos;
 
-        // variable-density forcing (SWEHR.c:1282-1290)
+        // variable-density forcing (SWEHR.c:1282-1290); the same factor in x and y
         R gmx = R(0), gmy = R(0);
-        if (gi > 0 && gi < nx - 1) gmx -= (rhos - rho) * gz * h * h / (R(2) * rho) * R(.5) * odx * (s_ts[e + EJ] - s_ts[e - EJ]);
-        if (gj > 0 && gj < ny - 1) gmy -= (rhos - rho) * gz * h * h / (R(2) * rho) * R(.5) * odx * (s_ts[e + 1] - s_ts[e - 1]);
+        const bool gx_in = gi > 0 && gi < nx - 1, gy_in = gj > 0 && gj < ny - 1;
+        if (gx_in || gy_in) {
+            const R fac = (rhos - rho) * gz * h * h / (R(2) * rho) * R(.5) * odx;
+            if (gx_in) gmx -= fac * (s_ts[e + EJ] - s_ts[e - EJ]);
+            if (gy_in) gmy -= fac * (s_ts[e + 1] - s_ts[e - 1]);
+        }
 
         // stability of the deposited layer (SWEHR.c:1292-1312): right/upper neighbour
-        // slopes of the previous step, left/lower ones of this step, own at the edges
+        // slopes of the previous step, left/lower ones of this step, own at the edges;
+        // cos(atan|x|) = 1/sqrt(1+x^2), sin(atan|x|) = |x|/sqrt(1+x^2)
         const R rho0 = R(kRhow * kPhi + kRhos * (1 - kPhi));
         const int du = gi < nx - 1 ? 1 : 0, dd = gi > 0 ? 1 : 0;
         const int ju = gj < ny - 1 ? 1 : 0, jd = gj > 0 ? 1 : 0;
         const R sx_up = du ? slope_x<R, ZJ>(s_zp, zi + 1, zj, gi + 1, nx, odx) : sxn;
         const R sx_dn = dd ? slope_x<R, ZJ>(s_z, zi - 1, zj, gi - 1, nx, odx) : sxn;
-        const R sy_up = ju ? slope_y<R, ZJ>(s_zp, zi, zj + 1, gj + 1, ny, odx) : syn;
-        const R sy_dn = jd ? slope_y<R, ZJ>(s_z, zi, zj - 1, gj - 1, ny, odx) : syn;
         const R dep = s_dep[e];
-        const R cy = cos_atan(syn);
+        const R cy = rsqrt(R(1) + syn * syn);
         const R load = rho0 * gz * dep + rho * gz * h;
         const R lhs = load * (fabs(syn) * cy);
         const R base = load * cy - rhow * gz * (dep + h);
         const R kfa = R(kFrictionAngle), kco = R(kCohesion);
-        R pgfx = dep * rho0 * gz * R(0.5) * odx *
-                 (s_dep[e + du * EJ] * cos_atan(sx_up) - s_dep[e - dd * EJ] * cos_atan(sx_dn));
+        const R pgf0 = dep * rho0 * gz * R(0.5) * odx;
+        R pgfx = pgf0 * (s_dep[e + du * EJ] * rsqrt(R(1) + sx_up * sx_up) -
+                         s_dep[e - dd * EJ] * rsqrt(R(1) + sx_dn * sx_dn));
         if (pgfx * sxn > R(0)) pgfx = R(-1) * fabs(pgfx);
         if (pgfx * sxn < R(0)) pgfx = fabs(pgfx);
         bool fail = lhs > (kco + base * kfa + pgfx);
         if (!fail) {
-            R pgfy = dep * rho0 * gz * R(0.5) * odx *
-                     (s_dep[e + ju] * cos_atan(sy_up) - s_dep[e - jd] * cos_atan(sy_dn));
+            const R sy_up = ju ? slope_y<R, ZJ>(s_zp, zi, zj + 1, gj + 1, ny, odx) : syn;
+            const R sy_dn = jd ? slope_y<R, ZJ>(s_z, zi, zj - 1, gj - 1, ny, odx) : syn;
+            R pgfy = pgf0 * (s_dep[e + ju] * rsqrt(R(1) + sy_up * sy_up) - s_dep[e - jd] * rsqrt(R(1) + sy_dn * sy_dn));
             if (pgfy * syn > R(0)) pgfy = R(-1) * fabs(pgfy);
             if (pgfy * syn < R(0)) pgfy = fabs(pgfy);
             fail = lhs > (kco + base * kfa + pgfy);
         }
 
         // debris-flow forcing (SWEHR.c:1314-1320)
-        R dfm = fmin(R(1), (ts - R(0.2)) / R(0.2));
+        R dfm = fmin(R(1), (ts - R(0.2)) * R(5));
         if (h < R(9) * h0 || ts < R(0.2)) dfm = R(0);
         const R bt = -(R(1) - R(kLambda)) * gz * h * kfa;
         const R btx = hu < R(0) ? R(-1) * bt : bt;
         const R bty = hv < R(0) ? R(-1) * bt : bt;
 
-        // class-invariant parts of the Hairsine-Rose sources (SWEHR.c:1329-1360)
+        // class-invariant parts of the Hairsine-Rose sources (SWEHR.c:1329-1378);
+        // depth normal to the bed h / cos(atan|slope|) = h * sqrt(1 + slope^2)
         const R totm_old = A.totm[c];
-        const R hvert = h * nrm;
+        const R hvert = h * (n2 * rn);
         const bool deep_v = hvert > h0;
-        const R mtstar = deep_v ? R(A.mtstar0) * (h0 / hvert) : R(A.mtstar0);
+        const R ihv = deep_v ? R(1) / hvert : R(0);
+        const R mtstar = deep_v ? R(A.mtstar0) * (h0 * ihv) : R(A.mtstar0);
         const R H = fmin(R(1), totm_old / mtstar);
-        const R shield = deep_v ? h0 / hvert : R(1);
+        const R shield = deep_v ? h0 * ihv : R(1);
         const R rough = R(A.roughness);
         const R hc = R(kHcFriction);
-        const R manning = (h <= hc) ? rough * R(pow(h / hc, R(kDepthExp))) : rough;
-        const R qq = hu * hu + hv * hv;
-        const R qm = sqrt(qq);
-        const R sf = manning * manning * qm * qm / R(pow(h, R(3.333)));
-        const R omega = rho * gz * sf * qm;
         const bool active = h > R(10) * hfilm && ts < R(0.6);
         const R oneh = R(1) - fmin(ts, R(0.99));
         const R hind = (oneh * oneh) * (oneh * oneh);
         const bool shutoff = h > R(9) * h0 && ts > R(kCthreshold);
         const bool shallow_v = h < R(9) * h0;
-        const R vshal = fmax(h - R(3) * h0, R(0));
-        const R s1 = R(kRhos / kRhow - 1);
-        const R cg = R(1.09) * s1 * gz;
-        const R e4den = (rhos - rho) / rhos * gz * h;
+        const R vshal = fmax(h - R(3) * h0, R(0)) * (R(1) / (R(6) * h0));
+        const R cg = R(1.09) * R(kRhos / kRhow - 1) * gz;
+        // rainsplash and stream-power factors; class k multiplies by P_k or M_k
+        R f1 = R(0), f2 = R(0), f3 = R(0), f4 = R(0), omega = R(0);
+        bool e2ok = false, e4ok = false;
+        if (active) {
+            const R manning = (h <= hc) ? rough * R(pow(h / hc, R(kDepthExp))) : rough;
+            const R qq = hu * hu + hv * hv;
+            const R qm = sqrt(qq);
+            const R sf = manning * manning * qq / R(pow(h, R(3.333)));
+            omega = rho * gz * sf * qm;
+            const R oneH = R(1) - H;
+            f1 = oneH * R(A.a) * Rr * shield;
+            e2ok = totm_old > R(1e-8);
+            e4ok = totm_old > R(1e-6);
+            const R itm = e2ok ? R(1) / totm_old : R(0);
+            f2 = H * itm * R(A.ad) * Rr * shield;
+            f3 = oneH * R(A.F) / R(A.J) * omega;
+            if (e4ok) f4 = (R(1) - R(kBetaBedload)) * H * itm * R(A.F) / ((rhos - rho) * inv_rhos * gz * h) * omega;
+        }
 
         // conservative update (SWEHR.c:1424-1441)
         const int fxl = il * TJ + jl, fxr = fxl + TJ;
@@ -463,13 +501,14 @@ __global__ __launch_bounds__(TI *TJ) void step_kernel(StepArgs<R> A)
         // sediment: flux divergence, sources and exchange per class (SWEHR.c:1454-1511)
         const R fx0l = s_fx[fxl], fx0r = s_fx[fxr], fy0l = s_fy[fyl], fy0r = s_fy[fyr];
         const int exl = e - EJ, exr = e + EJ, eyl = e - 1, eyr = e + 1;
-        const R kbed = R(1) / (rhos * (R(1) - R(kPhi)));
         const R rsp = rhos * (R(1) - R(kPhi));
+        const R kbed = R(1) / rsp;
+        const R hdt = R(0.5) * dt;
         R zn_c = s_z[zi * ZJ + zj];
         R totm = R(0), depn = R(0), chtemp = R(0);
         double sb_out = 0;
         for (int k = 0; k < S; k++) {
-            const R* sk = s_sed + k * EXT;
+            const R *sk = s_sed + k * EXT;
             const R sc = sk[e];
             const R fkxl = fx0l > R(0) ? fx0l * sk[exl] : fx0l * sc;
             const R fkxr = fx0r > R(0) ? fx0r * sc : fx0r * sk[exr];
@@ -485,20 +524,17 @@ __global__ __launch_bounds__(TI *TJ) void step_kernel(StepArgs<R> A)
             const R dk = R(A.ds[k]), pk = R(A.pf[k]);
             R e1 = R(0), e2 = R(0), e3 = R(0), e4 = R(0);
             if (active) {
-                e1 = (R(1) - H) * pk * R(A.a) * Rr;
-                if (totm_old > R(1e-8) && m > R(1e-8)) e2 = H * m / totm_old * R(A.ad) * Rr;
-                e1 = e1 * shield;
-                e2 = e2 * shield;
-                const R sp = (omega > R(0) && h > dk) ? omega : R(0);
-                e3 = (R(1) - H) * pk * R(A.F) / R(A.J) * sp;
-                if (totm_old > R(1e-6) && m > R(0))
-                    e4 = (R(1) - R(kBetaBedload)) * H * m / totm_old * R(A.F) / e4den * sp;
+                e1 = pk * f1;
+                if (e2ok && m > R(1e-8)) e2 = m * f2;
+                const bool ent = omega > R(0) && h > dk;
+                if (ent) e3 = pk * f3;
+                if (ent && e4ok && m > R(0)) e4 = m * f4;
             }
-            const R ak = R(13.95) * R(kNu) / dk;
-            R vf = sqrt(ak * ak + cg * dk) - ak;
+            const R ak = R(A.ak[k]);
+            R vf = sqrt(R(A.ak2[k]) + cg * dk) - ak;
             vf = vf * hind;
-            if (dk > R(0.0001) && shallow_v) vf = R(2e-3) + vshal * (vf - R(2e-3)) / (R(6) * h0);
-            if (dk > R(0.002) && shallow_v) vf = R(2e-3) + vshal * (vf - R(2e-3)) / (R(6) * h0);
+            if (dk > R(0.0001) && shallow_v) vf = R(2e-3) + vshal * (vf - R(2e-3));
+            if (dk > R(0.002) && shallow_v) vf = R(2e-3) + vshal * (vf - R(2e-3));
             if (shutoff) { e1 = e2 = e3 = e4 = R(0); vf = R(0); }
             // exchange with the deposited layer
             if (hn < R(10) * hfilm) {
@@ -508,7 +544,7 @@ __global__ __launch_bounds__(TI *TJ) void step_kernel(StepArgs<R> A)
             } else {
                 chtemp = u;
                 const R red = fmin(m, dt * (e2 + e4));
-                const R decay = exp(-vf / hn * (R(0.5) * dt));
+                const R decay = exp(-vf * hdt / hn);
                 u = decay * u;
                 u = u + dt * (e1 + e3) + red;
                 u = decay * u;
@@ -516,18 +552,18 @@ __global__ __launch_bounds__(TI *TJ) void step_kernel(StepArgs<R> A)
                 m = m + dt * (e1 + e3) - (u - chtemp);
                 bud[B_DET] += (double)(dt * (e1 + e3));
                 if (fail) {
-                    zn_c = zn_c - R(0.99) * m / rsp;
+                    zn_c = zn_c - R(0.99) * m * kbed;
                     u = u + R(0.99) * m;
                     totm = R(0);
                     m = R(0.01) * m;
                     depn = R(0);
                 }
             }
-            const R dh = (u - chtemp) / rsp;
+            const R dh = (u - chtemp) * kbed;
             hn = hn + dh;
             bud[B_EXCH] += (double)dh;
             totm += m;
-            depn += m / rhos;
+            depn += m * inv_rhos;
             A.ch[q][k * plane + c] = u;
             A.M[k * plane + c] = m;
         }
@@ -550,10 +586,12 @@ __global__ __launch_bounds__(TI *TJ) void step_kernel(StepArgs<R> A)
         if (hn > hfilm) {
             const R mn = (hn <= hc) ? rough * R(pow(hn / hc, R(kDepthExp))) : rough;
             const R sf2 = mn * mn * gz * sqrt(hun * hun + hvn * hvn) / R(pow(hn, R(2.333)));
-            hun = hun / (R(1) + dt * sf2);
-            hvn = hvn / (R(1) + dt * sf2);
-            v1 = hun / hn;
-            v2 = hvn / hn;
+            const R den = R(1) / (R(1) + dt * sf2);
+            hun = hun * den;
+            hvn = hvn * den;
+            const R ihn = R(1) / hn;
+            v1 = hun * ihn;
+            v2 = hvn * ihn;
         }
         const R speed = sqrt(v1 * v1 + v2 * v2);
         if (speed > A.maxvel[c]) A.maxvel[c] = speed;
@@ -561,7 +599,7 @@ __global__ __launch_bounds__(TI *TJ) void step_kernel(StepArgs<R> A)
         A.h[q][c] = hn;
         A.hu[q][c] = hun;
         A.hv[q][c] = hvn;
-        zn[c] = zn_c;
+        A.z[(p3 + 1) % 3][c] = zn_c;
         A.totm[c] = totm;
         A.dep[q][c] = depn;
 
@@ -579,7 +617,7 @@ __global__ __launch_bounds__(TI *TJ) void step_kernel(StepArgs<R> A)
         bud[B_DET] *= a2;
     }
 
-    // ---- per-block budget partials, accumulated across steps in the block's own slot
+    // ---- per-tile budget partials, accumulated across steps in the tile's own slot
     const int lane = tid & 63, wid = tid >> 6;
 #pragma unroll
     for (int k = 0; k < NB_TERMS; k++) {
@@ -598,6 +636,18 @@ __global__ __launch_bounds__(TI *TJ) void step_kernel(StepArgs<R> A)
             slot[2 * k + 1] = cc;
         }
     }
+}
+
+template <typename R, int TI, int TJ>
+__global__ __launch_bounds__(TI *TJ) void step_kernel(StepArgs<R> A)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    // XCD-aware tile order: consecutive block ids land on different XCDs, so give
+    // each XCD a contiguous band of tiles (their halos then share that XCD's L2).
+    const int b = blockIdx.x;
+    const int tile = (b & 7) * A.blocks_per_xcd + (b >> 3);
+    if (tile >= A.nblocks) return;
+    step_tile<R, TI, TJ>(A, tile, smem_raw);
 }
 
 // --------------------------------------------------------------------------- K3
@@ -619,8 +669,9 @@ __device__ __forceinline__ void derived(const StepArgs<R> &A, int p, long long c
     }
 }
 
+// End-of-step bookkeeping of one instance, run by one block.
 template <typename R>
-__global__ __launch_bounds__(256) void control_kernel(StepArgs<R> A)
+__device__ __forceinline__ void control_body(const StepArgs<R> &A)
 {
     DevCtrl *C = A.ctrl;
     if (!step_active(C)) return;
@@ -709,6 +760,51 @@ __global__ __launch_bounds__(256) void control_kernel(StepArgs<R> A)
         C->smax_bits = 0ull;
         if (!(tnew < A.tend) || C->status != 0) C->done = 1;
     }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void control_kernel(StepArgs<R> A)
+{
+    control_body(A);
+}
+
+// --------------------------------------------------------------------------- ensembles
+
+// Work map of an ensemble launch: entry k -> (instance, tile of that instance).  Block b
+// takes entry (b % 8) * per_xcd + b / 8, so each XCD works through a contiguous run
+// of entries (neighbouring tiles of one instance share that XCD's L2).
+struct TileRef {
+    int inst, tile;
+};
+
+template <typename R, int TI, int TJ>
+__global__ __launch_bounds__(TI *TJ) void batch_wave_kernel(const StepArgs<R> *args, const TileRef *map, int n,
+                                                            int per_xcd)
+{
+    const int b = blockIdx.x;
+    const int k = (b & 7) * per_xcd + (b >> 3);
+    if (k >= n) return;
+    const TileRef r = map[k];
+    wave_speed_tile<R, TI, TJ>(args[r.inst], r.tile);
+}
+
+template <typename R, int TI, int TJ>
+__global__ __launch_bounds__(TI *TJ) void batch_step_kernel(const StepArgs<R> *args, const TileRef *map, int n,
+                                                            int per_xcd)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int b = blockIdx.x;
+    const int k = (b & 7) * per_xcd + (b >> 3);
+    if (k >= n) return;
+    const TileRef r = map[k];
+    step_tile<R, TI, TJ>(args[r.inst], r.tile, smem_raw);
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void batch_control_kernel(const StepArgs<R> *args, const int *insts, int n)
+{
+    if ((int)blockIdx.x >= n) return;
+    control_body(args[insts[blockIdx.x]]);
 }
 
 }  // namespace swehr

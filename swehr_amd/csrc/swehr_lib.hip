@@ -69,11 +69,63 @@ void launch_step(int id, int blocks, size_t lds, hipStream_t s, const StepArgs<R
     }
 }
 
-constexpr size_t kLdsLimit = 64 * 1024;
-
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 }  // namespace
+
+namespace swehr {
+
+int tile_count() { return kNumTiles; }
+
+void tile_shape(int id, int *ti, int *tj)
+{
+    *ti = kTiles[id - 1].ti;
+    *tj = kTiles[id - 1].tj;
+}
+
+size_t tile_lds_bytes(int id, int prec, int S)
+{
+    return prec == SWEHR_FP64 ? tile_lds<double>(id, S) : tile_lds<float>(id, S);
+}
+
+template <typename R>
+static void batch_wave_t(int id, int blocks, hipStream_t s, const StepArgs<R> *a, const TileRef *map, int n, int per)
+{
+    switch (id) {
+    case 1: hipLaunchKernelGGL((batch_wave_kernel<R, 8, 32>), dim3(blocks), dim3(256), 0, s, a, map, n, per); break;
+    case 2: hipLaunchKernelGGL((batch_wave_kernel<R, 4, 64>), dim3(blocks), dim3(256), 0, s, a, map, n, per); break;
+    case 3: hipLaunchKernelGGL((batch_wave_kernel<R, 16, 16>), dim3(blocks), dim3(256), 0, s, a, map, n, per); break;
+    default: hipLaunchKernelGGL((batch_wave_kernel<R, 4, 32>), dim3(blocks), dim3(128), 0, s, a, map, n, per); break;
+    }
+}
+
+template <typename R>
+static void batch_step_t(int id, int blocks, size_t lds, hipStream_t s, const StepArgs<R> *a, const TileRef *map,
+                         int n, int per)
+{
+    switch (id) {
+    case 1: hipLaunchKernelGGL((batch_step_kernel<R, 8, 32>), dim3(blocks), dim3(256), lds, s, a, map, n, per); break;
+    case 2: hipLaunchKernelGGL((batch_step_kernel<R, 4, 64>), dim3(blocks), dim3(256), lds, s, a, map, n, per); break;
+    case 3: hipLaunchKernelGGL((batch_step_kernel<R, 16, 16>), dim3(blocks), dim3(256), lds, s, a, map, n, per); break;
+    default: hipLaunchKernelGGL((batch_step_kernel<R, 4, 32>), dim3(blocks), dim3(128), lds, s, a, map, n, per); break;
+    }
+}
+
+void launch_batch_wave(int id, int prec, int blocks, hipStream_t s, const void *args, const TileRef *map, int n,
+                       int per_xcd)
+{
+    if (prec == SWEHR_FP64) batch_wave_t(id, blocks, s, (const StepArgs<double> *)args, map, n, per_xcd);
+    else batch_wave_t(id, blocks, s, (const StepArgs<float> *)args, map, n, per_xcd);
+}
+
+void launch_batch_step(int id, int prec, int blocks, size_t lds, hipStream_t s, const void *args, const TileRef *map,
+                       int n, int per_xcd)
+{
+    if (prec == SWEHR_FP64) batch_step_t(id, blocks, lds, s, (const StepArgs<double> *)args, map, n, per_xcd);
+    else batch_step_t(id, blocks, lds, s, (const StepArgs<float> *)args, map, n, per_xcd);
+}
+
+}  // namespace swehr
 
 extern "C" int swehr_tile_count(void) { return kNumTiles; }
 
@@ -181,35 +233,6 @@ __global__ __launch_bounds__(256) void budget_kernel(StepArgs<R> A, double *out)
 }  // namespace swehr
 
 // ------------------------------------------------------------------ the handle
-
-struct swehr_handle {
-    swehr_params p;
-    int precision = SWEHR_FP64;
-    int device = 0;
-    int nx = 0, ny = 0, S = 0, py = 0;
-    long long plane = 0;
-    void *mem = nullptr;
-    size_t bytes = 0;
-    DevCtrl *ctrl = nullptr;   // device
-    DevCtrl *hctrl = nullptr;  // pinned host mirror
-    long long *hlimit = nullptr;
-    int *hflag = nullptr;
-    hipStream_t stream = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    int graph_steps = 16;
-    int tile = 0;
-    int tiles_i = 0, tiles_j = 0, nblocks = 0, blocks_per_xcd = 0, bud_slots = 0;
-    size_t lds = 0;
-    int k1_blocks = 0;
-    int stage_cap = 0;
-    double z0sum = 0;
-    double *scratch = nullptr;
-    double *partials = nullptr;
-    int budget_blocks = 0;
-    StepArgs<double> a64{};
-    StepArgs<float> a32{};
-};
 
 namespace {
 
@@ -358,6 +381,8 @@ void fill_args(swehr_handle *h, StepArgs<R> &A, const double *ds, const double *
     for (int k = 0; k < kMaxClasses; k++) {
         A.ds[k] = k < h->S ? ds[k] : 1.0;
         A.pf[k] = k < h->S ? pf[k] : 0.0;
+        A.ak[k] = 13.95 * kNu / A.ds[k];
+        A.ak2[k] = A.ak[k] * A.ak[k];
     }
 }
 
@@ -679,6 +704,77 @@ extern "C" int swehr_profile(swehr_handle *h, int nsteps, double *avg_ms, int *t
     HIPCHK(hipSetDevice(h->device));
     return h->precision == SWEHR_FP64 ? profile_impl(h, h->a64, nsteps, avg_ms, taken)
                                       : profile_impl(h, h->a32, nsteps, avg_ms, taken);
+}
+
+extern "C" int swehr_snapshot_bytes(swehr_handle *h, int64_t *bytes)
+{
+    if (!h || !bytes) return SWEHR_ERR_BAD_INPUT;
+    *bytes = (int64_t)h->bytes;
+    return SWEHR_OK;
+}
+
+extern "C" int swehr_snapshot(swehr_handle *h, void *host)
+{
+    if (!h || !host) return SWEHR_ERR_BAD_INPUT;
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipMemcpyAsync(host, h->mem, h->bytes, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return SWEHR_OK;
+}
+
+extern "C" int swehr_restore(swehr_handle *h, const void *host)
+{
+    if (!h || !host) return SWEHR_ERR_BAD_INPUT;
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipMemcpyAsync(h->mem, host, h->bytes, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return SWEHR_OK;
+}
+
+extern "C" int swehr_autotune(swehr_handle *h, int nsteps, int *best, double *ms_per_tile)
+{
+    if (!h || nsteps < 1) return SWEHR_ERR_BAD_INPUT;
+    HIPCHK(hipSetDevice(h->device));
+    void *save = nullptr;
+    if (hipMalloc(&save, h->bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        swehr_set_error("no device memory for the autotuner's state copy%s%s", "", "");
+        return SWEHR_ERR_OOM;
+    }
+    int rc = SWEHR_OK;
+    int best_id = h->tile;
+    double best_ms = 0;
+    const int orig = h->tile;
+    if (hipMemcpyAsync(save, h->mem, h->bytes, hipMemcpyDeviceToDevice, h->stream) != hipSuccess) rc = SWEHR_ERR_HIP;
+    for (int id = 1; id <= kNumTiles && rc == SWEHR_OK; id++) {
+        if (ms_per_tile) ms_per_tile[id - 1] = 0;
+        if (!choose_tile(h->precision, h->S, id)) continue;
+        rc = swehr_configure(h, -1, 0, id);
+        if (rc) break;
+        double ms[3];
+        int taken = 0;
+        rc = swehr_profile(h, nsteps, ms, &taken);
+        if (rc) break;
+        if (hipMemcpyAsync(h->mem, save, h->bytes, hipMemcpyDeviceToDevice, h->stream) != hipSuccess ||
+            hipStreamSynchronize(h->stream) != hipSuccess) {
+            rc = SWEHR_ERR_HIP;
+            break;
+        }
+        if (taken == 0) continue;
+        if (ms_per_tile) ms_per_tile[id - 1] = ms[1];
+        if (best_ms == 0 || ms[1] < best_ms) {
+            best_ms = ms[1];
+            best_id = id;
+        }
+    }
+    (void)hipFree(save);
+    if (rc) {
+        swehr_configure(h, -1, 0, orig);
+        return rc;
+    }
+    rc = swehr_configure(h, -1, 0, best_id);
+    if (best) *best = best_id;
+    return rc;
 }
 
 extern "C" int swehr_ack_print(swehr_handle *h)

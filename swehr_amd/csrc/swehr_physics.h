@@ -3,9 +3,10 @@
 //
 // Each function restates one piece of SWEHR.c (cited per function).  Expressions keep
 // the reference's operand order so the FP64 build stays within a few ulp of it; the
-// transcendental calls are strength-reduced where an identity is exact in real
-// arithmetic: pow(x,.5) -> sqrt(x), pow(x,2) -> x*x, cos(atan|s|) -> 1/sqrt(1+s^2),
-// sin(atan|s|) -> |s|/sqrt(1+s^2), and repeated calls with equal arguments are made once.
+// transcendental calls and divisions are strength-reduced where an identity is exact in
+// real arithmetic: pow(x,.5) -> sqrt(x), pow(x,2) -> x*x, a/b -> a*(1/b) with the
+// reciprocal shared, sqrt(g*d^2/(16g)) -> |d|/4, and repeated calls with equal
+// arguments are made once.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -35,15 +36,15 @@ __device__ __forceinline__ R face_flux(R hl, R hr, R qnl, R qnr, R qtl, R qtr, R
 {
     const R hfilm = R(kHfilm);
     R ul, vl, ur, vr;
-    if (hl > hfilm) { ul = qnl / hl; vl = qtl / hl; }
+    if (hl > hfilm) { const R r = R(1) / hl; ul = qnl * r; vl = qtl * r; }
     else { ul = R(0); vl = R(0); }
-    if (hr > hfilm) { ur = qnr / hr; vr = qtr / hr; }
+    if (hr > hfilm) { const R r = R(1) / hr; ur = qnr * r; vr = qtr * r; }
     else { ur = R(0); vr = R(0); }
     const R cl = sqrt(g * hl), cr = sqrt(g * hr);
     const R d = ul + R(2) * cl - ur - R(2) * cr;
-    const R hstar = d * d / (R(16) * g);
+    // h* = d^2 / (16 g), so c* = sqrt(g h*) = |d| / 4
     const R ustar = R(.5) * (ul + ur) + cl - cr;
-    const R cstar = sqrt(g * hstar);
+    const R cstar = fabs(d) * R(0.25);
     R sl = ul - cl, sm = ustar, sr = ur + cr;
     if (ustar - cstar < sl) sl = ustar - cstar;
     if (ustar + cstar > sr) sr = ustar + cstar;
@@ -56,9 +57,10 @@ __device__ __forceinline__ R face_flux(R hl, R hr, R qnl, R qnr, R qtl, R qtr, R
         a1 = hl * ul * ul + hg * hl * hl;
     }
     if (sl <= R(0) && sr >= R(0)) {
-        a0 = (sr * (hl * ul) - sl * (hr * ur) + sl * sr * (hr - hl)) / (sr - sl);
-        a1 = (sr * (hl * ul * ul + hg * hl * hl) - sl * (hr * ur * ur + hg * hr * hr) + sl * sr * (hr * ur - hl * ul)) /
-             (sr - sl);
+        const R iw = R(1) / (sr - sl);
+        a0 = (sr * (hl * ul) - sl * (hr * ur) + sl * sr * (hr - hl)) * iw;
+        a1 = (sr * (hl * ul * ul + hg * hl * hl) - sl * (hr * ur * ur + hg * hr * hr) + sl * sr * (hr * ur - hl * ul)) *
+             iw;
     }
     if (sr <= R(0)) {
         a0 = hr * ur;
@@ -79,13 +81,12 @@ template <typename R>
 __device__ __forceinline__ R face_speed(R hl, R hr, R qnl, R qnr, R g)
 {
     const R hfilm = R(kHfilm);
-    const R ul = (hl > hfilm) ? qnl / hl : R(0);
-    const R ur = (hr > hfilm) ? qnr / hr : R(0);
+    const R ul = (hl > hfilm) ? qnl * (R(1) / hl) : R(0);
+    const R ur = (hr > hfilm) ? qnr * (R(1) / hr) : R(0);
     const R cl = sqrt(g * hl), cr = sqrt(g * hr);
     const R d = ul + R(2) * cl - ur - R(2) * cr;
-    const R hstar = d * d / (R(16) * g);
     const R ustar = R(.5) * (ul + ur) + cl - cr;
-    const R cstar = sqrt(g * hstar);
+    const R cstar = fabs(d) * R(0.25);
     R sl = ul - cl, sm = ustar, sr = ur + cr;
     if (ustar - cstar < sl) sl = ustar - cstar;
     if (ustar + cstar > sr) sr = ustar + cstar;

@@ -63,6 +63,25 @@ def lib():
     L.swehr_tile_shape.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.swehr_get_stream.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
     L.swehr_profile.argtypes = [C.c_void_p, C.c_int, P, C.POINTER(C.c_int)]
+    L.swehr_snapshot_bytes.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+    L.swehr_snapshot.argtypes = [C.c_void_p, C.c_void_p]
+    L.swehr_restore.argtypes = [C.c_void_p, C.c_void_p]
+    L.swehr_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), P]
+    PP = C.POINTER(C.POINTER(C.c_double))
+    L.swehr_batch_create.argtypes = [C.c_int, C.POINTER(_abi.Params), PP, PP, PP, PP, PP, C.c_int, C.c_int,
+                                     C.POINTER(C.c_void_p)]
+    L.swehr_batch_advance.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_int)]
+    L.swehr_batch_handle.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
+    L.swehr_batch_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), P]
+    L.swehr_batch_destroy.argtypes = [C.c_void_p]
+    L.swehr_server_create.argtypes = [C.c_int64, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+    L.swehr_server_submit.argtypes = [C.c_void_p, C.POINTER(_abi.Params), P, P, P, P, P, C.POINTER(C.c_int64)]
+    L.swehr_server_pump.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                    C.POINTER(C.c_int)]
+    L.swehr_server_next_done.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(_abi.StepInfo)]
+    L.swehr_server_request.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_void_p)]
+    L.swehr_server_release.argtypes = [C.c_void_p, C.c_int64]
+    L.swehr_server_destroy.argtypes = [C.c_void_p]
     _lib = L
     return L
 
@@ -102,10 +121,14 @@ class Solver:
     """One GPU instance of a ``swehr_amd.inputs.Case``."""
 
     def __init__(self, case, precision: int = _abi.FP64, pause_on_print: bool = False, steps_per_graph: int = 0,
-                 tile: int = 0):
+                 tile: int = 0, _borrow=None):
         L = lib()
         self.case = case
         self.precision = precision
+        self._owned = _borrow is None
+        if _borrow is not None:
+            self._h = _borrow
+            return
         self._p = _abi.params_from_case(case)
         topo, ks, ds, pf, rain = _abi.case_arrays(case)
         h = C.c_void_p()
@@ -185,9 +208,29 @@ class Solver:
         check("swehr_get_stream", lib().swehr_get_stream(self._h, C.byref(s)))
         return int(s.value or 0)
 
+    def snapshot(self) -> np.ndarray:
+        n = C.c_int64(0)
+        check("swehr_snapshot_bytes", lib().swehr_snapshot_bytes(self._h, C.byref(n)))
+        buf = np.empty(n.value, dtype=np.uint8)
+        check("swehr_snapshot", lib().swehr_snapshot(self._h, buf.ctypes.data_as(C.c_void_p)))
+        return buf
+
+    def restore(self, buf: np.ndarray):
+        buf = np.ascontiguousarray(buf, dtype=np.uint8)
+        check("swehr_restore", lib().swehr_restore(self._h, buf.ctypes.data_as(C.c_void_p)))
+
+    def autotune(self, nsteps: int = 10):
+        """Time every tile shape for nsteps steps (state restored after each), keep the
+        fastest; returns (best_id, ms_per_tile)."""
+        best = C.c_int(0)
+        ms = np.zeros(lib().swehr_tile_count())
+        check("swehr_autotune", lib().swehr_autotune(self._h, int(nsteps), C.byref(best), dptr(ms)))
+        return best.value, ms
+
     def close(self):
         if getattr(self, "_h", None):
-            lib().swehr_destroy(self._h)
+            if self._owned:
+                lib().swehr_destroy(self._h)
             self._h = None
 
     def __enter__(self):
@@ -213,3 +256,129 @@ def water_residual(b: np.ndarray) -> float:
 def sediment_residual(b: np.ndarray) -> float:
     """Sediment budget residual (kg): suspended + deposited minus detachment and inflow."""
     return float(b[4] + b[5] - (b[8] - b[7]))
+
+
+def _ptr_array(arrays):
+    arr = (C.POINTER(C.c_double) * len(arrays))()
+    for i, a in enumerate(arrays):
+        arr[i] = dptr(a) if a is not None else C.cast(None, C.POINTER(C.c_double))
+    return arr
+
+
+class Batch:
+    """An ensemble of cases stepped together on the GPU (one precision, shared tile
+    shape, per-instance dt)."""
+
+    def __init__(self, cases, precision: int = _abi.FP64, tile: int = 0):
+        L = lib()
+        self.cases = list(cases)
+        self.precision = precision
+        n = len(self.cases)
+        self._params = (_abi.Params * n)(*[_abi.params_from_case(c) for c in self.cases])
+        self._keep = [_abi.case_arrays(c) for c in self.cases]
+        cols = list(zip(*self._keep))  # topo, ks, ds, pf, rain
+        ptrs = [_ptr_array(col) for col in cols]
+        h = C.c_void_p()
+        check("swehr_batch_create", L.swehr_batch_create(n, self._params, ptrs[0], ptrs[1], ptrs[2], ptrs[3], ptrs[4],
+                                                         int(precision), int(tile), C.byref(h)))
+        self._b = h
+        self._views = {}
+
+    def advance(self, n: int = 1 << 62) -> int:
+        left = C.c_int(0)
+        check("swehr_batch_advance", lib().swehr_batch_advance(self._b, int(n), C.byref(left)))
+        return left.value
+
+    def member(self, i: int) -> Solver:
+        if i not in self._views:
+            h = C.c_void_p()
+            check("swehr_batch_handle", lib().swehr_batch_handle(self._b, int(i), C.byref(h)))
+            self._views[i] = Solver(self.cases[i], self.precision, _borrow=h)
+        return self._views[i]
+
+    def autotune(self, nsteps: int = 5):
+        best = C.c_int(0)
+        ms = np.zeros(lib().swehr_tile_count())
+        check("swehr_batch_autotune", lib().swehr_batch_autotune(self._b, int(nsteps), C.byref(best), dptr(ms)))
+        return best.value, ms
+
+    def close(self):
+        if getattr(self, "_b", None):
+            for v in self._views.values():
+                v._h = None
+            lib().swehr_batch_destroy(self._b)
+            self._b = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching solve server (requests admitted by slots and memory)."""
+
+    def __init__(self, mem_bytes: int, max_slots: int, precision: int = _abi.FP32, tile: int = 0):
+        h = C.c_void_p()
+        check("swehr_server_create", lib().swehr_server_create(int(mem_bytes), int(max_slots), int(precision),
+                                                               int(tile), C.byref(h)))
+        self._s = h
+        self.precision = precision
+        self.cases = {}
+
+    def submit(self, case) -> int:
+        p = _abi.params_from_case(case)
+        topo, ks, ds, pf, rain = _abi.case_arrays(case)
+        rid = C.c_int64(0)
+        rc = lib().swehr_server_submit(self._s, C.byref(p), dptr(topo), dptr(ks) if ks is not None else None,
+                                       dptr(ds), dptr(pf), dptr(rain), C.byref(rid))
+        check("swehr_server_submit", rc)
+        self.cases[rid.value] = case
+        return rid.value
+
+    def pump(self, steps: int = 16):
+        r, q, f = C.c_int(0), C.c_int(0), C.c_int(0)
+        check("swehr_server_pump", lib().swehr_server_pump(self._s, int(steps), C.byref(r), C.byref(q), C.byref(f)))
+        return r.value, q.value, f.value
+
+    def next_done(self):
+        rid = C.c_int64(0)
+        info = _abi.StepInfo()
+        rc = lib().swehr_server_next_done(self._s, C.byref(rid), C.byref(info))
+        if rc == _abi.ERR_BUSY:
+            return None
+        check("swehr_server_next_done", rc)
+        return rid.value, info.as_dict()
+
+    def result(self, rid: int) -> Solver:
+        h = C.c_void_p()
+        check("swehr_server_request", lib().swehr_server_request(self._s, int(rid), C.byref(h)))
+        return Solver(self.cases[rid], self.precision, _borrow=h)
+
+    def release(self, rid: int):
+        check("swehr_server_release", lib().swehr_server_release(self._s, int(rid)))
+        self.cases.pop(rid, None)
+
+    def close(self):
+        if getattr(self, "_s", None):
+            lib().swehr_server_destroy(self._s)
+            self._s = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,159 @@
+"""Ensembles, serving, checkpoints and the autotuner on the GPU.
+
+The state of an instance does not depend on which launch its tiles ride in, on the
+tile shape, or on what else is on the GPU: every cell is computed by the same code
+from the same inputs.  So an instance stepped in an ensemble or inside the server
+must equal, bit for bit, the same case stepped alone; a restored checkpoint must
+continue exactly as the original; the autotuner must leave the state untouched.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle_lib import Oracle
+from parity_util import compare_states, worst
+from swehr_amd import _abi
+from swehr_amd import solver as sv
+from swehr_amd.inputs import config_c4_member, config_c5_request, read_case
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+FIELDS = ("depth", "uh", "vh", "topo", "vinf", "totm", "dep", "maxvel", "maxdepth")
+
+
+def _golden(name):
+    return read_case(os.path.join(GOLD, name, "inputs"), name=name)
+
+
+def _mix(tend=90.0):
+    rng = np.random.default_rng(21)
+    cases = [_golden("transmissive"), _golden("closed_basin"), _golden("inflow_mix")]
+    for k in range(3):
+        c = config_c4_member(rng, k, tend=tend, scale_max=3.0)
+        cases.append(c)
+    return cases
+
+
+def _same(a, b, case):
+    for f in FIELDS:
+        np.testing.assert_array_equal(a.field(f), b.field(f), err_msg="%s %s" % (case.name, f))
+    for k in range(case.nclass):
+        np.testing.assert_array_equal(a.field("ch", k + 1), b.field("ch", k + 1))
+        np.testing.assert_array_equal(a.field("m", k + 1), b.field("m", k + 1))
+    sa, fa = a.stage()
+    sb, fb = b.stage()
+    np.testing.assert_array_equal(sa, sb)
+    np.testing.assert_array_equal(fa, fb)
+
+
+@pytest.mark.parametrize("precision", [_abi.FP64, _abi.FP32], ids=["fp64", "fp32"])
+def test_ensemble_members_equal_alone_runs(precision):
+    cases = _mix()
+    print("ensemble of", [(c.name, c.nx, c.ny, c.nclass) for c in cases], flush=True)
+    with sv.Batch(cases, precision=precision) as b:
+        left = b.advance()
+        assert left == 0
+        for i, c in enumerate(cases):
+            m = b.member(i)
+            mi = m.info()
+            with sv.Solver(c, precision=precision) as s:
+                s.run()
+                si = s.info()
+                assert si["steps"] == mi["steps"] and si["t"] == mi["t"], c.name
+                _same(m, s, c)
+        # per-instance dt: the members took different numbers of steps
+        assert len({b.member(i).info()["steps"] for i in range(len(cases))}) > 1
+
+
+def test_ensemble_fp64_matches_oracle():
+    cases = [_golden("transmissive"), _golden("closed_basin")]
+    with sv.Batch(cases, precision=_abi.FP64) as b:
+        b.advance()
+        for i, c in enumerate(cases):
+            o = Oracle(c)
+            o.run()
+            errs = compare_states(b.member(i).state(), o.state(), c.nclass)
+            k, e = worst(errs)
+            assert e <= 1e-7, (c.name, k, e)
+
+
+def test_ensemble_step_limit_and_resume():
+    cases = _mix()
+    with sv.Batch(cases, precision=_abi.FP64) as b:
+        b.advance(37)
+        steps = [b.member(i).info()["steps"] for i in range(len(cases))]
+        assert all(s == 37 or b.member(i).info()["done"] for i, s in enumerate(steps))
+        b.advance()
+        c = cases[4]
+        with sv.Solver(c) as s:
+            s.run()
+            _same(b.member(4), s, c)
+
+
+def test_server_continuous_batching_results_equal_alone():
+    rng = np.random.default_rng(31)
+    reqs = [config_c5_request(rng, k, sizes=(32, 48, 64), weights=(1, 1, 1), tmin=30.0, tmax=90.0) for k in range(6)]
+    per = max(sv.instance_bytes(c.nx, c.ny, c.nclass, _abi.FP32) for c in reqs)
+    with sv.Server(mem_bytes=int(2.5 * per), max_slots=4, precision=_abi.FP32) as srv:
+        ids = [srv.submit(c) for c in reqs]
+        done = {}
+        max_running = 0
+        for _ in range(10000):
+            running, queued, finished = srv.pump(16)
+            max_running = max(max_running, running)
+            while True:
+                d = srv.next_done()
+                if d is None:
+                    break
+                done[d[0]] = d[1]
+            if len(done) == len(reqs):
+                break
+        assert len(done) == len(reqs)
+        sizes = [sv.instance_bytes(c.nx, c.ny, c.nclass, _abi.FP32) for c in reqs]
+        assert max_running <= min(4, int(2.5 * per) // min(sizes))  # slots and memory budget
+        assert max_running < len(reqs)
+        for rid, c in zip(ids, reqs):
+            r = srv.result(rid)
+            with sv.Solver(c, precision=_abi.FP32) as s:
+                s.run()
+                assert s.info()["steps"] == done[rid]["steps"]
+                _same(r, s, c)
+            srv.release(rid)
+
+
+def test_server_rejects_request_larger_than_budget():
+    rng = np.random.default_rng(32)
+    c = config_c5_request(rng, 0, sizes=(64,), weights=(1,), tmin=30, tmax=40)
+    with sv.Server(mem_bytes=1 << 20, max_slots=2, precision=_abi.FP32) as srv:
+        with pytest.raises(sv.SwehrError) as ei:
+            srv.submit(c)
+        assert ei.value.code == _abi.ERR_BUSY
+
+
+def test_snapshot_restore_continues_identically():
+    c = _golden("transmissive")
+    with sv.Solver(c) as s:
+        s.run(100)
+        snap = s.snapshot()
+        s.run(200)
+        a = {f: s.field(f) for f in FIELDS}
+        s.restore(snap)
+        assert s.info()["steps"] == 100
+        s.run(200)
+        for f in FIELDS:
+            np.testing.assert_array_equal(s.field(f), a[f])
+
+
+def test_autotune_keeps_state_and_picks_a_fitting_tile():
+    c = _golden("closed_basin")
+    with sv.Solver(c) as s, sv.Solver(c) as ref:
+        s.run(50)
+        best, ms = s.autotune(5)
+        assert 1 <= best <= len(ms) and ms[best - 1] > 0
+        assert ms[best - 1] == min(m for m in ms if m > 0)
+        s.run(150)
+        ref.run(200)
+        for f in FIELDS:
+            np.testing.assert_array_equal(s.field(f), ref.field(f))

@@ -1,0 +1,57 @@
+"""The drop-in program (swehr_amd/bin/swehr) against the oracle's program on the
+golden inputs: same stdout lines and the same 17 output files, numerically equal to
+the print precision of the reference's formats (rasters "%10.10lf", series "%lf").
+The oracle's program itself reproduces the compiled reference byte for byte
+(test_oracle_golden.py), so this pins the drop-in outputs to the reference's."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from golden.cases import OUTPUT_FILES, stdout_lines
+from oracle_lib import RUN_PATH
+from swehr_amd.solver import DRIVER
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _numbers(path):
+    with open(path) as fh:
+        return np.array([float(x) for x in fh.read().split()])
+
+
+def _run(prog, src, dst):
+    os.makedirs(dst)
+    for fn in os.listdir(src):
+        shutil.copy(os.path.join(src, fn), dst)
+    p = subprocess.run([prog, dst], capture_output=True, text=True, timeout=600, env=dict(os.environ, OMP_NUM_THREADS="1"))
+    assert p.returncode == 0, p.stderr
+    return stdout_lines(p.stdout)
+
+
+@pytest.mark.parametrize("name", ["shipped", "inflow_mix", "rain_plane", "transmissive", "cfl_abort"])
+def test_driver_outputs_match_oracle_program(name, tmp_path):
+    src = os.path.join(GOLD, name, "inputs")
+    print("case", name, flush=True)
+    a = _run(RUN_PATH, src, str(tmp_path / "oracle"))
+    b = _run(DRIVER, src, str(tmp_path / "gpu"))
+    assert len(a) == len(b)
+    for la, lb in zip(a, b):
+        if la == lb:
+            continue
+        fa = [float(x) for x in la.replace("!!! CFL Error at t=", "").split()]
+        fb = [float(x) for x in lb.replace("!!! CFL Error at t=", "").split()]
+        np.testing.assert_allclose(fb, fa, rtol=1e-6, atol=2e-6)
+    for fn in OUTPUT_FILES:
+        pa, pb = str(tmp_path / "oracle" / fn), str(tmp_path / "gpu" / fn)
+        assert os.path.exists(pa) == os.path.exists(pb), fn
+        va, vb = _numbers(pa), _numbers(pb)
+        assert va.shape == vb.shape, fn
+        assert np.array_equal(np.isnan(va), np.isnan(vb)), fn
+        m = ~np.isnan(va)
+        tol = 2e-6 if fn in ("stage.txt", "saveflow.txt") else 1e-8
+        np.testing.assert_allclose(vb[m], va[m], rtol=1e-7, atol=tol, err_msg=fn)
