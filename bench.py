@@ -116,6 +116,160 @@ def traffic_for(key: str):
     return None if v is None else v.get("bytes_per_launch")
 
 
+def rel_l2(a, b):
+    import numpy as np
+
+    m = np.isfinite(b) & np.isfinite(a)
+    den = float(np.linalg.norm(b[m]))
+    return float(np.linalg.norm(a[m] - b[m]) / den) if den > 0 else 0.0
+
+
+def budget_errors(b):
+    from swehr_amd import solver as sv
+
+    wscale = max(abs(b[k]) for k in (0, 1, 2, 3, 9, 10))
+    sscale = max(max(abs(b[k]) for k in (4, 5, 7, 8)), 1e-30)
+    return abs(sv.water_residual(b)) / wscale, abs(sv.sediment_residual(b)) / sscale
+
+
+def bench_precision(hip, grid: int, t_check: float, window: int):
+    """Precision form on the channel-network case: FP64 and FP32 from the same initial
+    state to t_check (errors of the FP32 result against FP64), then a timed window of
+    each at that state (throughput and the step kernel's roofline)."""
+    from swehr_amd import _abi
+    from swehr_amd import solver as sv
+    from swehr_amd.inputs import config_c3
+
+    case = config_c3(n=grid)
+    S, cells = case.nclass, case.nx * case.ny
+    case.tend = t_check
+    out = {"workload": "%dx%d channel network, S=%d, triangular hyetograph (peak 120 mm/h), per-cell Ks"
+                       % (grid, grid, S), "t_check_s": t_check}
+    res = {}
+    for name, prec in (("fp64", _abi.FP64), ("fp32", _abi.FP32)):
+        with sv.Solver(case, precision=prec) as s:
+            s.run()
+            st, _ = s.stage()
+            S_fields = {"depth": s.field("depth"), "speed": s.field("speed"), "stage": st}
+            S_fields["sed"] = [s.field("sed", k + 1) for k in range(S)]
+            res[name] = (S_fields, s.info(), s.budget())
+    a, b = res["fp32"][0], res["fp64"][0]
+    n = min(a["stage"].shape[1], b["stage"].shape[1])
+    out["fp32_vs_fp64"] = {
+        "rel_l2_depth": rel_l2(a["depth"], b["depth"]),
+        "rel_l2_speed": rel_l2(a["speed"], b["speed"]),
+        "rel_l2_concentration_per_class": [rel_l2(x, y) for x, y in zip(a["sed"], b["sed"])],
+        "rel_l2_outlet_hydrograph": rel_l2(a["stage"][1, :n], b["stage"][1, :n]),
+        "rel_l2_outlet_sedigraph": rel_l2(a["stage"][3, :n], b["stage"][3, :n]),
+        "steps": [res["fp32"][1]["steps"], res["fp64"][1]["steps"]]}
+    for name in ("fp64", "fp32"):
+        we, se = budget_errors(res[name][2])
+        out["mass_balance_rel_%s" % name] = {"water": we, "sediment": se}
+    # timed windows, continuing beyond t_check
+    case.tend = 3600.0
+    for name, prec, w in (("fp64", _abi.FP64, 8), ("fp32", _abi.FP32, 4)):
+        with sv.Solver(case, precision=prec) as s:
+            s.run(int(4 * t_check))
+            best, _ = s.autotune(5)
+            n1, ms, _, info = timed_run(hip, s, window)
+            kms, _ = s.profile(10)
+            bcell = bytes_per_cell(S, w)
+            out[name] = {"value": cells * S * n1 / (ms * 1e-3), "ms_per_step": ms / max(n1, 1),
+                         "step_kernel_ms": float(kms[1]),
+                         "roofline_frac": cells * bcell / (float(kms[1]) * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                         "bytes_per_cell": bcell, "t_s": info["t"], "tile": best}
+    out["fp32_over_fp64"] = out["fp32"]["value"] / out["fp64"]["value"]
+    return out
+
+
+def bench_batched(hip, count: int, window: int, scale_max: float):
+    """Ensemble of hillslopes: FP32 in one ensemble (every stage one launch over all
+    instances) against plain FP64 instances run one after another, same steps."""
+    from swehr_amd import _abi
+    from swehr_amd import solver as sv
+    from swehr_amd.inputs import config_c4
+
+    cases = config_c4(count=count, scale_max=scale_max)
+    cells = sum(c.nx * c.ny for c in cases)
+    S = cases[0].nclass
+    warm = 20
+    with sv.Batch(cases, precision=_abi.FP32) as b:
+        best, tile_ms = b.autotune(3)
+        b.advance(warm)
+        e0, e1 = hip.event(), hip.event()
+        hip.sync()
+        t0 = time.perf_counter()
+        b.advance(window)
+        hip.sync()
+        batch_s = time.perf_counter() - t0
+        steps = sum(b.member(i).info()["steps"] - warm for i in range(len(cases)))
+    batch_value = sum(c.nx * c.ny for c in cases) * S * window / batch_s
+    plain_s = 0.0
+    plain_cc = 0
+    for c in cases:
+        with sv.Solver(c, precision=_abi.FP64) as s:
+            s.run(warm)
+            n, ms, wall, _ = timed_run(hip, s, window)
+            plain_s += wall
+            plain_cc += c.nx * c.ny * S * n
+    plain_value = plain_cc / plain_s
+    return {"workload": "%d hillslopes, %d cells total, S=%d, %d steps each after %d warm-up steps"
+                        % (len(cases), cells, S, window, warm),
+            "fp32_batched": {"value": batch_value, "wall_s": batch_s, "tile": best,
+                             "autotune_ms_per_step": [round(float(x), 4) for x in tile_ms]},
+            "fp64_plain_sequential": {"value": plain_value, "wall_s": plain_s},
+            "ratio": batch_value / plain_value, "member_steps": steps}
+
+
+def bench_serving(hip, nreq: int, rate: float, time_scale: float, mem_gib: float, slots: int, seed: int = 5):
+    """Poisson stream of solve requests into the continuous-batching server (FP32 state,
+    FP64 budgets); latency = completion - arrival (wall clock)."""
+    import numpy as np
+
+    from swehr_amd import _abi
+    from swehr_amd import solver as sv
+    from swehr_amd.inputs import config_c5_request
+
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for k in range(nreq):
+        c = config_c5_request(rng, k)
+        c.tend = max(10.0, c.tend * time_scale)
+        reqs.append(c)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    lat, work = {}, 0.0
+    with sv.Server(mem_bytes=int(mem_gib * 2 ** 30), max_slots=slots, precision=_abi.FP32) as srv:
+        t0 = time.perf_counter()
+        sub = {}
+        nxt = 0
+        while len(lat) < nreq:
+            now = time.perf_counter() - t0
+            while nxt < nreq and arrivals[nxt] <= now:
+                rid = srv.submit(reqs[nxt])
+                sub[rid] = (nxt, t0 + arrivals[nxt])
+                nxt += 1
+            running, queued, finished = srv.pump(16)
+            while True:
+                d = srv.next_done()
+                if d is None:
+                    break
+                rid, info = d
+                k, ta = sub[rid]
+                lat[k] = time.perf_counter() - ta
+                c = reqs[k]
+                work += c.nx * c.ny * c.nclass * info["steps"]
+                srv.release(rid)
+            if running == 0 and queued == 0 and nxt < nreq:
+                time.sleep(max(0.0, min(0.01, t0 + arrivals[nxt] - time.perf_counter())))
+        wall = time.perf_counter() - t0
+    L = np.array([lat[k] for k in range(nreq)])
+    return {"workload": "%d requests, Poisson rate %.2f/s, sizes 128^2-2048^2 (weights .4/.3/.15/.1/.05), "
+                        "S 3-10, durations U[600,3600] s x %.2f" % (nreq, rate, time_scale),
+            "throughput_cell_class_steps_per_s": work / wall, "wall_s": wall,
+            "latency_p50_s": float(np.percentile(L, 50)), "latency_p99_s": float(np.percentile(L, 99)),
+            "latency_mean_s": float(L.mean()), "mem_budget_gib": mem_gib, "slots": slots}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +283,19 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--steps-per-graph", type=int, default=16)
     ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--forms", default="precision,batched,serving",
+                    help="extra measurements after the headline: comma list of precision, batched, serving, or none")
+    ap.add_argument("--c3-grid", type=int, default=4096)
+    ap.add_argument("--c3-check", type=float, default=900.0, help="simulated seconds for the FP32-vs-FP64 check")
+    ap.add_argument("--c3-window", type=int, default=100)
+    ap.add_argument("--c4-count", type=int, default=256)
+    ap.add_argument("--c4-window", type=int, default=100)
+    ap.add_argument("--c4-scale-max", type=float, default=16.0)
+    ap.add_argument("--c5-requests", type=int, default=40)
+    ap.add_argument("--c5-rate", type=float, default=4.0)
+    ap.add_argument("--c5-time-scale", type=float, default=0.05)
+    ap.add_argument("--c5-mem-gib", type=float, default=64.0)
+    ap.add_argument("--c5-slots", type=int, default=32)
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (HBM counter calibration)")
     args = ap.parse_args()
@@ -166,6 +333,10 @@ def main():
     n_first, ms_first, _, _ = timed_run(hip, s, 20)
     s.run(max(0, args.warmup - n_first))
     log("warmup done: t=%.1f s after %d steps" % (s.info()["t"], s.info()["steps"]))
+    tile_ms = None
+    if not args.tile:
+        best, tile_ms = s.autotune(8)
+        log("autotuned tile %d: %s ms per step kernel" % (best, [round(float(x), 4) for x in tile_ms]))
     n, ms, wall, info = timed_run(hip, s, args.steps)
     value = cells * S * n / (ms * 1e-3)
     log("timed %d steps: %.3f ms (wall %.3f s) -> %.4e cell-class-steps/s, t=%.1f" % (n, ms, wall, value, info["t"]))
@@ -193,6 +364,19 @@ def main():
                          % (ncpu, args.grid, args.grid, el, cells * S * n_first / (ms_first * 1e-3))}
         log("cpu baseline: %s" % cpu)
 
+    forms = {}
+    wanted = [f for f in args.forms.split(",") if f and f != "none"]
+    if "precision" in wanted:
+        forms["precision_c3"] = bench_precision(hip, args.c3_grid, args.c3_check, args.c3_window)
+        log("precision: %s" % json.dumps(forms["precision_c3"]))
+    if "batched" in wanted:
+        forms["batched_c4"] = bench_batched(hip, args.c4_count, args.c4_window, args.c4_scale_max)
+        log("batched: %s" % json.dumps(forms["batched_c4"]))
+    if "serving" in wanted:
+        forms["serving_c5"] = bench_serving(hip, args.c5_requests, args.c5_rate, args.c5_time_scale, args.c5_mem_gib,
+                                            args.c5_slots)
+        log("serving: %s" % json.dumps(forms["serving_c5"]))
+
     wres = sv.water_residual(bud)
     wscale = max(abs(bud[k]) for k in (0, 1, 2, 3, 9, 10))
     out = {
@@ -211,12 +395,14 @@ def main():
         "config": {"workload": "plain %s step, %dx%d cells, dx 1 m, S=%d, 80 mm/h rain, per-cell Ks"
                                % (args.precision.upper(), args.grid, args.grid, S),
                    "grid": [case.nx, case.ny], "classes": S, "precision": args.precision,
-                   "t_start_s": None, "t_end_s": info["t"], "tile": args.tile or "auto"},
+                   "t_end_s": info["t"], "tile": args.tile or "autotuned",
+                   "autotune_step_kernel_ms": None if tile_ms is None else [round(float(x), 4) for x in tile_ms]},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels_ms": {"wave_speed": round(float(kms[0]), 4), "step": round(step_ms, 4),
                        "control": round(float(kms[2]), 4), "profiled_steps": taken},
         "water_budget_rel_residual": abs(wres) / wscale if wscale else 0.0,
+        "forms": forms,
     }
     print(json.dumps(out))
 

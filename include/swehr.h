@@ -109,6 +109,7 @@ typedef struct swehr_step_info {
     double dt;         /* dt of the last step                                  */
     double courant;    /* Smax*dt/dx of the last step (SWEHR.c:1412)           */
     double rain;       /* rain rate R of the last step (m/s)                   */
+    double t_prev;     /* time at the start of the last step                   */
     int64_t steps;     /* steps taken since create                             */
     int32_t done;      /* 1 once t >= tend or the run stopped                  */
     int32_t print_due; /* 1: the last step crossed a print interval (SWEHR.c:1679); */
@@ -245,13 +246,19 @@ int swehr_server_next_done(swehr_server *s, int64_t *id, swehr_step_info *info);
 int swehr_server_request(swehr_server *s, int64_t id, swehr_handle **h);
 /* Free a finished request's device memory. */
 int swehr_server_release(swehr_server *s, int64_t id);
+/* Device memory held by admitted requests and the budget (bytes). */
+int swehr_server_stats(swehr_server *s, int64_t *mem_used, int64_t *mem_budget);
 int swehr_server_destroy(swehr_server *s);
 
 /* Last error message of the calling thread (empty when none). */
 const char *swehr_last_error(void);
 
-/* Device bytes held by one instance of these sizes (for admission control). */
+/* Upper bound of the device bytes of one instance of these sizes (a per-cell Ks
+ * raster, 4096 rain entries and a full 30000-record outlet series included). */
 int64_t swehr_instance_bytes(int nx, int ny, int nclass, int precision);
+/* Exact device bytes of an instance with these parameters (what swehr_create
+ * allocates; ks_raster: 1 if a per-cell Ks raster is given). */
+int64_t swehr_instance_bytes_for(const swehr_params *p, int precision, int ks_raster);
 
 /* Release the instance and its device memory. */
 int swehr_destroy(swehr_handle *h);

@@ -64,7 +64,7 @@ struct oracle {
     double *fx, *fy, *sx, *sy, *gmx, *gmy, *dfm, *btx, *bty, *gzn, *E, *vf;
     char *fail;
     /* driver state */
-    double t, dt, courant, R, stageinterval, printinterval;
+    double t, dt, courant, R, stageinterval, printinterval, t_prev;
     long steps;
     int stagecnt, status, done;
     long fail_events; /* cells flagged unstable, summed over steps (diagnostic) */
@@ -631,6 +631,7 @@ int oracle_step(oracle_t *o, int *print_due)
     update_pass(o);
     nsum(&o->bud[2], &o->bud_c[2], dt * o->R * (double)nx * ny * o->p.dx * o->p.dx);
     o->bud[11] += 1;
+    o->t_prev = o->t;
     if (cfl) { o->t = 2 * o->p.tend; o->status = SWEHR_ERR_CFL; }
     o->t += dt;
     o->steps++;
@@ -655,6 +656,7 @@ void oracle_info(const oracle_t *o, swehr_step_info *info)
     info->dt = o->dt;
     info->courant = o->courant;
     info->rain = o->R;
+    info->t_prev = o->t_prev;
     info->steps = o->steps;
     info->done = o->done;
     info->print_due = 0;

@@ -35,7 +35,7 @@ class Params(C.Structure):
 
 class StepInfo(C.Structure):
     _fields_ = [("t", C.c_double), ("dt", C.c_double), ("courant", C.c_double), ("rain", C.c_double),
-                ("steps", C.c_int64), ("done", C.c_int32), ("print_due", C.c_int32),
+                ("t_prev", C.c_double), ("steps", C.c_int64), ("done", C.c_int32), ("print_due", C.c_int32),
                 ("stage_count", C.c_int32), ("status", C.c_int32)]
 
     def as_dict(self):

@@ -58,10 +58,8 @@ int main(int argc, char **argv)
     for (;;) {
         rc = swehr_advance(h, (int64_t)1 << 40, &info);
         if (rc == SWEHR_ERR_CFL) {
-            /* the reference prints this before its last step (SWEHR.c:1413); the step's
-             * start time is t - dt unless the stop moved t to 2*tend + dt */
-            t_before = info.t - info.dt;
-            if (t_before >= 2 * in.p.tend) t_before = info.t - info.dt - 2 * in.p.tend;
+            /* the reference prints this before its last step (SWEHR.c:1413) */
+            t_before = info.t_prev;
             printf("!!! CFL Error at t=%lf\n", t_before);
         } else if (rc != SWEHR_OK) {
             return fail("advance", rc);

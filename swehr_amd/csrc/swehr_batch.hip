@@ -513,7 +513,7 @@ extern "C" int swehr_server_submit(swehr_server *s, const swehr_params *p, const
         swehr_set_error("request's class count does not fit the server's tile shape%s%s", "", "");
         return SWEHR_ERR_BAD_INPUT;
     }
-    const int64_t need = swehr_instance_bytes(p->nx, p->ny, p->nclass, s->precision);
+    const int64_t need = swehr_instance_bytes_for(p, s->precision, ks != nullptr);
     if (need < 0 || need > s->mem_budget) {
         swehr_set_error("request needs more device memory than the server's budget%s%s", "", "");
         return SWEHR_ERR_BUSY;
@@ -576,6 +576,7 @@ extern "C" int swehr_server_pump(swehr_server *s, int steps, int *running, int *
                     info.dt = c.dt;
                     info.courant = c.courant;
                     info.rain = c.rain;
+                    info.t_prev = c.t_prev;
                     info.steps = c.steps;
                     info.done = c.done;
                     info.print_due = c.print_due;
@@ -629,6 +630,14 @@ extern "C" int swehr_server_release(swehr_server *s, int64_t id)
     s->ids.erase(it->second);
     swehr_destroy(it->second);
     s->finished.erase(it);
+    return SWEHR_OK;
+}
+
+extern "C" int swehr_server_stats(swehr_server *s, int64_t *mem_used, int64_t *mem_budget)
+{
+    if (!s) return SWEHR_ERR_BAD_INPUT;
+    if (mem_used) *mem_used = s->mem_used;
+    if (mem_budget) *mem_budget = s->mem_budget;
     return SWEHR_OK;
 }
 

@@ -42,6 +42,7 @@ enum BudgetTerm { B_EXCH = 0, B_CLAMP = 1, B_DET = 2, B_BWATER = 3, B_BSED = 4, 
 
 struct alignas(64) DevCtrl {
     double t, dt, courant, rain;
+    double t_prev;  // time at the start of the last step (the CFL message prints it)
     double stageinterval, printinterval, printint, rain_sum;
     double rain_comp;
     unsigned long long smax_bits;
@@ -140,7 +141,7 @@ __device__ __forceinline__ R wave_speed_cell(const StepArgs<R> &A, const R *h, c
     if (j != 0 && j != ny - 1) sy = R(0.5) * odx * (zp[c + 1] - zp[c - 1]);
     else if (j == ny - 1) sy = odx * (zp[c] - zp[j > 0 ? c - 1 : c]);
     else sy = odx * (zp[ny > 1 ? c + 1 : c] - zp[c]);
-    const R g = R(kG) * rsqrt(sx * sx + sy * sy + R(1));
+    const R g = R(kG) * mrsqrt(sx * sx + sy * sy + R(1));
     const R hc = h[c], huc = hu[c], hvc = hv[c];
     R smax = R(0), s;
     {
@@ -265,7 +266,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
 
     double *s_red = reinterpret_cast<double *>(smem_raw);
     R *s_h = reinterpret_cast<R *>(s_red + NB_TERMS * 16);
-    R *s_hu = s_h + EXT, *s_hv = s_hu + EXT, *s_ts = s_hv + EXT, *s_dep = s_ts + EXT;
+    R *s_u = s_h + EXT, *s_v = s_u + EXT, *s_ts = s_v + EXT, *s_dep = s_ts + EXT;
     R *s_z = s_dep + EXT, *s_zp = s_z + ZEXT;
     R *s_fx = s_zp + ZEXT, *s_fy = s_fx + 3 * NFX;
     R *s_sed = s_fy + 3 * NFY;
@@ -289,15 +290,22 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         if (gj < 0 && A.bndry[1] == 1) hv = -hv;
         const bool inflow_ghost = gj >= ny && A.inflow;
         if (gj >= ny && !A.inflow && A.bndry[3] == 1) hv = -hv;
-        R ts = R(0);
+        R ts = R(0), u = R(0), v = R(0);
         if (inflow_ghost) {
             h = R(in.h);
             hu = R(0);
             hv = R(in.hv);
             const R sd = R(in.c) / h;
             for (int k = 0; k < S; k++) s_sed[k * EXT + e] = sd;
+            if (h > hfilm) {
+                const R ih = R(1) / h;
+                u = hu * ih;
+                v = hv * ih;
+            }
         } else if (h > hfilm) {
             const R ih = R(1) / h;
+            u = hu * ih;
+            v = hv * ih;
             for (int k = 0; k < S; k++) {
                 R sd = chin[k * plane + c] * ih;
                 ts += sd * inv_rhos;
@@ -308,8 +316,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             for (int k = 0; k < S; k++) s_sed[k * EXT + e] = R(0);
         }
         s_h[e] = h;
-        s_hu[e] = hu;
-        s_hv[e] = hv;
+        s_u[e] = u;
+        s_v[e] = v;
         s_ts[e] = ts;
         s_dep[e] = depin[c];
     }
@@ -332,39 +340,51 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         const int zi = gi - (i0 - 2), zj = gj - (j0 - 2);
         const R a = slope_x<R, ZJ>(s_zp, zi, zj, gi, nx, odx);
         const R bb = slope_y<R, ZJ>(s_zp, zi, zj, gj, ny, odx);
-        return g * rsqrt(a * a + bb * bb + R(1));
+        return g * mrsqrt(a * a + bb * bb + R(1));
     };
 
     // ---- phase 2: face fluxes of the tile, each computed once
     const int il = tid / TJ, jl = tid - (tid / TJ) * TJ;
-    auto xface = [&](int fi, int fj) {
-        const int gil = i0 + fi - 1, gj = j0 + fj;
-        if (gj >= ny || gil >= nx) return;
+    // x face between ext cells el (left) and el + EJ, with g and cl of the left cell
+    auto xface_g = [&](int fi, int fj, R gz, R cl) {
         const int el = fi * EJ + fj + 1, er = el + EJ;
-        const R gz = gz_lag(gil < 0 ? 0 : gil, gj);
         R f0, f1, f2;
-        face_flux(s_h[el], s_h[er], s_hu[el], s_hu[er], s_hv[el], s_hv[er], gz, f0, f1, f2);
+        face_flux_v(s_h[el], s_h[er], s_u[el], s_u[er], s_v[el], s_v[er], cl, msqrt(gz * s_h[er]), gz, f0, f1, f2);
         const int f = fi * TJ + fj;
         s_fx[f] = f0;
         s_fx[NFX + f] = f1;
         s_fx[2 * NFX + f] = f2;
     };
-    auto yface = [&](int fi, int fj) {
-        const int gi = i0 + fi, gjl = j0 + fj - 1;
-        if (gi >= nx || gjl >= ny) return;
+    // y face between ext cells el (lower) and el + 1; normal velocity v (SWEHR.c:1203)
+    auto yface_g = [&](int fi, int fj, R gz, R cl) {
         const int el = (fi + 1) * EJ + fj, er = el + 1;
-        const R gz = gz_lag(gi, gjl < 0 ? 0 : gjl);
         R f0, f1, f2;
-        face_flux(s_h[el], s_h[er], s_hv[el], s_hv[er], s_hu[el], s_hu[er], gz, f0, f1, f2);
+        face_flux_v(s_h[el], s_h[er], s_v[el], s_v[er], s_u[el], s_u[er], cl, msqrt(gz * s_h[er]), gz, f0, f1, f2);
         const int f = fi * (TJ + 1) + fj;
         s_fy[f] = f0;
         s_fy[NFY + f] = f2;      // hu flux (tangential)
         s_fy[2 * NFY + f] = f1;  // hv flux (normal), SWEHR.c:1216-1218
     };
-    xface(il + 1, jl);
-    yface(il, jl + 1);
-    if (il == 0) xface(0, jl);
-    if (jl == 0) yface(il, 0);
+    {
+        // the cell's own faces i+1/2 and j+1/2 share its lagged GZ and celerity
+        const int gi = i0 + il, gj = j0 + jl;
+        if (gi < nx && gj < ny) {
+            const R gz = gz_lag(gi, gj);
+            const R cl = msqrt(gz * s_h[(il + 1) * EJ + jl + 1]);
+            xface_g(il + 1, jl, gz, cl);
+            yface_g(il, jl + 1, gz, cl);
+        }
+        // faces on the tile's low edges belong to the cell outside (or to the ghost,
+        // which takes the GZ of the first cell, SWEHR.c:1054-1088)
+        if (il == 0 && gj < ny && i0 <= nx) {
+            const R gz = gz_lag(i0 > 0 ? i0 - 1 : 0, gj);
+            xface_g(0, jl, gz, msqrt(gz * s_h[jl + 1]));
+        }
+        if (jl == 0 && gi < nx && j0 <= ny) {
+            const R gz = gz_lag(gi, j0 > 0 ? j0 - 1 : 0);
+            yface_g(il, 0, gz, msqrt(gz * s_h[(il + 1) * EJ]));
+        }
+    }
     __syncthreads();
 
     // ---- phase 3: sources and update of the tile's own cells
@@ -374,7 +394,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         const long long c = (long long)gi * py + gj;
         const int e = (il + 1) * EJ + (jl + 1);
         const int zi = il + 2, zj = jl + 2;
-        const R h = s_h[e], hu = s_hu[e], hv = s_hv[e];
+        const R h = s_h[e], hu = huin[c], hv = hvin[c];
         const R ts = s_ts[e];
         const R dt = R(dtd), Rr = R(Rd);
         const R h0 = R(A.h0);
@@ -383,10 +403,10 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         const R sxn = slope_x<R, ZJ>(s_z, zi, zj, gi, nx, odx);
         const R syo = slope_y<R, ZJ>(s_zp, zi, zj, gj, ny, odx);
         const R syn = slope_y<R, ZJ>(s_z, zi, zj, gj, ny, odx);
-        const R gx = g * sxn * rsqrt(sxn * sxn + syo * syo + R(1));
+        const R gx = g * sxn * mrsqrt(sxn * sxn + syo * syo + R(1));
         const R sxs = -gx * h;
         const R n2 = sxn * sxn + syn * syn + R(1);
-        const R rn = rsqrt(n2);
+        const R rn = mrsqrt(n2);
         const R gy = g * syn * rn;
         const R sys = -gy * h;
         const R gz = g * rn;
@@ -406,66 +426,64 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
 
         // stability of the deposited layer (SWEHR.c:1292-1312): right/upper neighbour
         // slopes of the previous step, left/lower ones of this step, own at the edges;
-        // cos(atan|x|) = 1/sqrt(1+x^2), sin(atan|x|) = |x|/sqrt(1+x^2)
+        // cos(atan|x|) = 1/msqrt(1+x^2), sin(atan|x|) = |x|/msqrt(1+x^2)
         const R rho0 = R(kRhow * kPhi + kRhos * (1 - kPhi));
         const int du = gi < nx - 1 ? 1 : 0, dd = gi > 0 ? 1 : 0;
         const int ju = gj < ny - 1 ? 1 : 0, jd = gj > 0 ? 1 : 0;
         const R sx_up = du ? slope_x<R, ZJ>(s_zp, zi + 1, zj, gi + 1, nx, odx) : sxn;
         const R sx_dn = dd ? slope_x<R, ZJ>(s_z, zi - 1, zj, gi - 1, nx, odx) : sxn;
         const R dep = s_dep[e];
-        const R cy = rsqrt(R(1) + syn * syn);
+        const R cy = mrsqrt(R(1) + syn * syn);
         const R load = rho0 * gz * dep + rho * gz * h;
-        const R lhs = load * (fabs(syn) * cy);
+        const R lhs = load * (mfabs(syn) * cy);
         const R base = load * cy - rhow * gz * (dep + h);
         const R kfa = R(kFrictionAngle), kco = R(kCohesion);
         const R pgf0 = dep * rho0 * gz * R(0.5) * odx;
-        R pgfx = pgf0 * (s_dep[e + du * EJ] * rsqrt(R(1) + sx_up * sx_up) -
-                         s_dep[e - dd * EJ] * rsqrt(R(1) + sx_dn * sx_dn));
-        if (pgfx * sxn > R(0)) pgfx = R(-1) * fabs(pgfx);
-        if (pgfx * sxn < R(0)) pgfx = fabs(pgfx);
+        R pgfx = pgf0 * (s_dep[e + du * EJ] * mrsqrt(R(1) + sx_up * sx_up) -
+                         s_dep[e - dd * EJ] * mrsqrt(R(1) + sx_dn * sx_dn));
+        if (pgfx * sxn > R(0)) pgfx = R(-1) * mfabs(pgfx);
+        if (pgfx * sxn < R(0)) pgfx = mfabs(pgfx);
         bool fail = lhs > (kco + base * kfa + pgfx);
         if (!fail) {
             const R sy_up = ju ? slope_y<R, ZJ>(s_zp, zi, zj + 1, gj + 1, ny, odx) : syn;
             const R sy_dn = jd ? slope_y<R, ZJ>(s_z, zi, zj - 1, gj - 1, ny, odx) : syn;
-            R pgfy = pgf0 * (s_dep[e + ju] * rsqrt(R(1) + sy_up * sy_up) - s_dep[e - jd] * rsqrt(R(1) + sy_dn * sy_dn));
-            if (pgfy * syn > R(0)) pgfy = R(-1) * fabs(pgfy);
-            if (pgfy * syn < R(0)) pgfy = fabs(pgfy);
+            R pgfy = pgf0 * (s_dep[e + ju] * mrsqrt(R(1) + sy_up * sy_up) - s_dep[e - jd] * mrsqrt(R(1) + sy_dn * sy_dn));
+            if (pgfy * syn > R(0)) pgfy = R(-1) * mfabs(pgfy);
+            if (pgfy * syn < R(0)) pgfy = mfabs(pgfy);
             fail = lhs > (kco + base * kfa + pgfy);
         }
 
         // debris-flow forcing (SWEHR.c:1314-1320)
-        R dfm = fmin(R(1), (ts - R(0.2)) * R(5));
+        R dfm = mfmin(R(1), (ts - R(0.2)) * R(5));
         if (h < R(9) * h0 || ts < R(0.2)) dfm = R(0);
         const R bt = -(R(1) - R(kLambda)) * gz * h * kfa;
         const R btx = hu < R(0) ? R(-1) * bt : bt;
         const R bty = hv < R(0) ? R(-1) * bt : bt;
 
         // class-invariant parts of the Hairsine-Rose sources (SWEHR.c:1329-1378);
-        // depth normal to the bed h / cos(atan|slope|) = h * sqrt(1 + slope^2)
+        // depth normal to the bed h / cos(atan|slope|) = h * msqrt(1 + slope^2)
         const R totm_old = A.totm[c];
         const R hvert = h * (n2 * rn);
         const bool deep_v = hvert > h0;
         const R ihv = deep_v ? R(1) / hvert : R(0);
         const R mtstar = deep_v ? R(A.mtstar0) * (h0 * ihv) : R(A.mtstar0);
-        const R H = fmin(R(1), totm_old / mtstar);
+        const R H = mfmin(R(1), totm_old / mtstar);
         const R shield = deep_v ? h0 * ihv : R(1);
         const R rough = R(A.roughness);
-        const R hc = R(kHcFriction);
         const bool active = h > R(10) * hfilm && ts < R(0.6);
-        const R oneh = R(1) - fmin(ts, R(0.99));
+        const R oneh = R(1) - mfmin(ts, R(0.99));
         const R hind = (oneh * oneh) * (oneh * oneh);
         const bool shutoff = h > R(9) * h0 && ts > R(kCthreshold);
         const bool shallow_v = h < R(9) * h0;
-        const R vshal = fmax(h - R(3) * h0, R(0)) * (R(1) / (R(6) * h0));
+        const R vshal = mfmax(h - R(3) * h0, R(0)) * (R(1) / (R(6) * h0));
         const R cg = R(1.09) * R(kRhos / kRhow - 1) * gz;
         // rainsplash and stream-power factors; class k multiplies by P_k or M_k
         R f1 = R(0), f2 = R(0), f3 = R(0), f4 = R(0), omega = R(0);
         bool e2ok = false, e4ok = false;
         if (active) {
-            const R manning = (h <= hc) ? rough * R(pow(h / hc, R(kDepthExp))) : rough;
             const R qq = hu * hu + hv * hv;
-            const R qm = sqrt(qq);
-            const R sf = manning * manning * qq / R(pow(h, R(3.333)));
+            const R qm = msqrt(qq);
+            const R sf = manning2_over_hpow(h, rough, R(3.333)) * qq;
             omega = rho * gz * sf * qm;
             const R oneH = R(1) - H;
             f1 = oneH * R(A.a) * Rr * shield;
@@ -493,7 +511,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             const double vin = A.vinf[c];
             const double zf = vin / (A.thetas - A.theta0);
             const double hd = (double)hn;
-            const double infl = fmin(hd - kHfilm, dtd * ks * (zf + A.hf + hd) / zf);
+            const double infl = mfmin(hd - kHfilm, dtd * ks * (zf + A.hf + hd) / zf);
             A.vinf[c] = vin + infl;
             hn = R(hd - infl);
         }
@@ -531,7 +549,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
                 if (ent && e4ok && m > R(0)) e4 = m * f4;
             }
             const R ak = R(A.ak[k]);
-            R vf = sqrt(R(A.ak2[k]) + cg * dk) - ak;
+            R vf = msqrt(R(A.ak2[k]) + cg * dk) - ak;
             vf = vf * hind;
             if (dk > R(0.0001) && shallow_v) vf = R(2e-3) + vshal * (vf - R(2e-3));
             if (dk > R(0.002) && shallow_v) vf = R(2e-3) + vshal * (vf - R(2e-3));
@@ -543,8 +561,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
                 u = R(0);
             } else {
                 chtemp = u;
-                const R red = fmin(m, dt * (e2 + e4));
-                const R decay = exp(-vf * hdt / hn);
+                const R red = mfmin(m, dt * (e2 + e4));
+                const R decay = mexp(-vf * hdt / hn);
                 u = decay * u;
                 u = u + dt * (e1 + e3) + red;
                 u = decay * u;
@@ -584,8 +602,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         // implicit friction (SWEHR.c:1553-1565) and maxima (SWEHR.c:1587-1588)
         R v1 = R(0), v2 = R(0);
         if (hn > hfilm) {
-            const R mn = (hn <= hc) ? rough * R(pow(hn / hc, R(kDepthExp))) : rough;
-            const R sf2 = mn * mn * gz * sqrt(hun * hun + hvn * hvn) / R(pow(hn, R(2.333)));
+            const R sf2 = manning2_over_hpow(hn, rough, R(2.333)) * gz * msqrt(hun * hun + hvn * hvn);
             const R den = R(1) / (R(1) + dt * sf2);
             hun = hun * den;
             hvn = hvn * den;
@@ -593,7 +610,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             v1 = hun * ihn;
             v2 = hvn * ihn;
         }
-        const R speed = sqrt(v1 * v1 + v2 * v2);
+        const R speed = msqrt(v1 * v1 + v2 * v2);
         if (speed > A.maxvel[c]) A.maxvel[c] = speed;
         if (hn > A.maxdepth[c]) A.maxdepth[c] = hn;
         A.h[q][c] = hn;
@@ -752,6 +769,7 @@ __device__ __forceinline__ void control_body(const StepArgs<R> &A)
         neumaier(rs, rc, dt * Rr * ((double)A.nx * A.ny) * A.dx * A.dx);
         C->rain_sum = rs;
         C->rain_comp = rc;
+        C->t_prev = t;
         C->t = tnew;
         C->dt = dt;
         C->courant = courant;

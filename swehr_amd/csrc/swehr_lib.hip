@@ -451,6 +451,25 @@ int check_params(const swehr_params *p)
 
 }  // namespace
 
+extern "C" int64_t swehr_instance_bytes_for(const swehr_params *p, int precision, int ks_raster)
+{
+    if (!p || p->nx < 1 || p->ny < 1 || p->nclass < 1 || p->nclass > kMaxClasses || p->rnum < 1) return -1;
+    swehr_handle h;
+    h.p = *p;
+    h.precision = precision;
+    h.nx = p->nx;
+    h.ny = p->ny;
+    h.S = p->nclass;
+    h.py = pick_pitch(p->ny);
+    h.plane = (long long)h.nx * h.py;
+    h.stage_cap = stage_capacity(p->tend);
+    const int tile = choose_tile(precision, h.S, 0);
+    if (tile == 0) return -1;
+    geometry(&h, tile);
+    return (int64_t)(precision == SWEHR_FP64 ? plan<double>(&h, nullptr, ks_raster != 0)
+                                             : plan<float>(&h, nullptr, ks_raster != 0));
+}
+
 extern "C" int64_t swehr_instance_bytes(int nx, int ny, int nclass, int precision)
 {
     swehr_handle h;
@@ -623,6 +642,7 @@ static void fill_info(const DevCtrl &c, swehr_step_info *info)
     info->dt = c.dt;
     info->courant = c.courant;
     info->rain = c.rain;
+    info->t_prev = c.t_prev;
     info->steps = c.steps;
     info->done = c.done;
     info->print_due = c.print_due;

@@ -12,6 +12,20 @@
 
 namespace swehr {
 
+// Math helpers overloaded for both precisions, so FP32 code never promotes to FP64.
+__device__ __forceinline__ double msqrt(double x) { return __builtin_sqrt(x); }
+__device__ __forceinline__ float msqrt(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ double mrsqrt(double x) { return rsqrt(x); }
+__device__ __forceinline__ float mrsqrt(float x) { return rsqrtf(x); }
+__device__ __forceinline__ double mfabs(double x) { return __builtin_fabs(x); }
+__device__ __forceinline__ float mfabs(float x) { return __builtin_fabsf(x); }
+__device__ __forceinline__ double mfmin(double a, double b) { return __builtin_fmin(a, b); }
+__device__ __forceinline__ float mfmin(float a, float b) { return __builtin_fminf(a, b); }
+__device__ __forceinline__ double mfmax(double a, double b) { return __builtin_fmax(a, b); }
+__device__ __forceinline__ float mfmax(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ __forceinline__ double mexp(double x) { return exp(x); }
+__device__ __forceinline__ float mexp(float x) { return expf(x); }
+
 // constants of SWEHR.c:489-494, 632-650
 constexpr double kG = 9.81;
 constexpr double kHfilm = 1e-5;
@@ -27,24 +41,21 @@ constexpr double kBetaBedload = 0.5;
 constexpr double kHcFriction = 0.003;
 constexpr double kDepthExp = -0.33;
 
-// HLL flux across one face, SWEHR.c:327-409.  (qn, qt): normal and tangential
-// discharge.  Writes mass, normal- and tangential-momentum flux; returns the largest
-// |wave speed|.  The flux branches are sequential ifs: the last true one wins (so
+// HLL flux across one face, SWEHR.c:327-409, from the face's left/right depths and
+// velocities (un normal, ut tangential; the reference's qn/h and qt/h, zero where
+// h <= hfilm, SWEHR.c:333-352) and the celerities cl = sqrt(g hl), cr = sqrt(g hr).
+// Writes mass, normal- and tangential-momentum flux; returns the largest |wave
+// speed|.  The flux branches are sequential ifs: the last true one wins (so
 // hl = hr = 0 yields zero flux, not the 0/0 of the middle branch).
 template <typename R>
-__device__ __forceinline__ R face_flux(R hl, R hr, R qnl, R qnr, R qtl, R qtr, R g, R &f0, R &f1, R &f2)
+__device__ __forceinline__ R face_flux_v(R hl, R hr, R ul, R ur, R vl, R vr, R cl, R cr, R g, R &f0, R &f1,
+                                         R &f2)
 {
     const R hfilm = R(kHfilm);
-    R ul, vl, ur, vr;
-    if (hl > hfilm) { const R r = R(1) / hl; ul = qnl * r; vl = qtl * r; }
-    else { ul = R(0); vl = R(0); }
-    if (hr > hfilm) { const R r = R(1) / hr; ur = qnr * r; vr = qtr * r; }
-    else { ur = R(0); vr = R(0); }
-    const R cl = sqrt(g * hl), cr = sqrt(g * hr);
     const R d = ul + R(2) * cl - ur - R(2) * cr;
-    // h* = d^2 / (16 g), so c* = sqrt(g h*) = |d| / 4
+    // h* = d^2 / (16 g), so c* = msqrt(g h*) = |d| / 4
     const R ustar = R(.5) * (ul + ur) + cl - cr;
-    const R cstar = fabs(d) * R(0.25);
+    const R cstar = mfabs(d) * R(0.25);
     R sl = ul - cl, sm = ustar, sr = ur + cr;
     if (ustar - cstar < sl) sl = ustar - cstar;
     if (ustar + cstar > sr) sr = ustar + cstar;
@@ -70,10 +81,32 @@ __device__ __forceinline__ R face_flux(R hl, R hr, R qnl, R qnr, R qtl, R qtr, R
     f1 = a1;
     f2 = (sm >= R(0)) ? a0 * vl : a0 * vr;
     R s = R(0);
-    if (fabs(sl) > s) s = fabs(sl);
-    if (fabs(sr) > s) s = fabs(sr);
-    if (fabs(sm) > s) s = fabs(sm);
+    if (mfabs(sl) > s) s = mfabs(sl);
+    if (mfabs(sr) > s) s = mfabs(sr);
+    if (mfabs(sm) > s) s = mfabs(sm);
     return s;
+}
+
+// Natural log and exp used for the friction powers: the library versions in FP64,
+// the hardware-based fast versions in FP32.
+__device__ __forceinline__ double fast_log(double x) { return log(x); }
+__device__ __forceinline__ float fast_log(float x) { return __logf(x); }
+__device__ __forceinline__ double fast_exp(double x) { return exp(x); }
+__device__ __forceinline__ float fast_exp(float x) { return __expf(x); }
+
+// ln(0.003): log of the critical depth of the depth-dependent Manning law (SWEHR.c:490)
+constexpr double kLnHcFriction = -5.809142990314028;
+
+// n^2 / h^p for Manning's n of SWEHR.c:1356-1357 (n0 (h/0.003)^-0.33 below 0.003 m,
+// n0 above): n0^2 exp(-p ln h - 0.66 (ln h - ln 0.003)) or n0^2 exp(-p ln h).  One log
+// and one exp replace pow(h/0.003, -0.33) and pow(h, p) (SWEHR.c:1360, 1559).
+template <typename R>
+__device__ __forceinline__ R manning2_over_hpow(R h, R n0, R p)
+{
+    const R lh = fast_log(h);
+    R ex = -p * lh;
+    if (h <= R(kHcFriction)) ex += R(2 * kDepthExp) * (lh - R(kLnHcFriction));
+    return n0 * n0 * fast_exp(ex);
 }
 
 // Wave-speed bound of the same face without the fluxes (SWEHR.c:333-376, 404-408).
@@ -83,19 +116,19 @@ __device__ __forceinline__ R face_speed(R hl, R hr, R qnl, R qnr, R g)
     const R hfilm = R(kHfilm);
     const R ul = (hl > hfilm) ? qnl * (R(1) / hl) : R(0);
     const R ur = (hr > hfilm) ? qnr * (R(1) / hr) : R(0);
-    const R cl = sqrt(g * hl), cr = sqrt(g * hr);
+    const R cl = msqrt(g * hl), cr = msqrt(g * hr);
     const R d = ul + R(2) * cl - ur - R(2) * cr;
     const R ustar = R(.5) * (ul + ur) + cl - cr;
-    const R cstar = fabs(d) * R(0.25);
+    const R cstar = mfabs(d) * R(0.25);
     R sl = ul - cl, sm = ustar, sr = ur + cr;
     if (ustar - cstar < sl) sl = ustar - cstar;
     if (ustar + cstar > sr) sr = ustar + cstar;
     if (hl <= hfilm) { sl = ur - R(2) * cr; sm = sl; sr = ur + cr; }
     if (hr <= hfilm) { sl = ul - cl; sr = ul + R(2) * cl; sm = sr; }
     R s = R(0);
-    if (fabs(sl) > s) s = fabs(sl);
-    if (fabs(sr) > s) s = fabs(sr);
-    if (fabs(sm) > s) s = fabs(sm);
+    if (mfabs(sl) > s) s = mfabs(sl);
+    if (mfabs(sr) > s) s = mfabs(sr);
+    if (mfabs(sm) > s) s = mfabs(sm);
     return s;
 }
 

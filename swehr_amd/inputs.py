@@ -233,34 +233,34 @@ def fractal_topography(nx: int, ny: int, dx: float, relief: float, tilt: float, 
     return f + tilt * dx * j[None, :]
 
 
-def carve_channels(z: np.ndarray, dx: float, rng: np.random.Generator, threshold_cells: int = 400,
-                   depth: float = 0.5) -> np.ndarray:
-    """Incise a channel network: cells whose D8 flow accumulation (steepest descent on
-    the surface) exceeds ``threshold_cells`` are lowered by up to ``depth`` metres,
-    growing with the square root of the accumulation.  A small seeded jitter breaks
-    ties on flat patches."""
+def carve_channels(z: np.ndarray, dx: float, rng: np.random.Generator, n_channels: int = 0, depth: float = 1.5,
+                   width: float = 0.0) -> np.ndarray:
+    """Incise a seeded network of downslope channels: n_channels meandering centre
+    lines i_c(j) (random phase, amplitude, wavelength and a random-walk drift), joined
+    in pairs towards the outlet row, each carving a Gaussian valley of the given
+    depth (m) and width (cells) that deepens downslope."""
     nx, ny = z.shape
-    zz = z + 1e-6 * rng.random(z.shape)
-    order = np.argsort(-zz, axis=None)
-    acc = np.ones(nx * ny)
-    flat = zz.reshape(-1)
-    for idx in order:
-        i, j = divmod(int(idx), ny)
-        best, bi = 0.0, -1
-        for di in (-1, 0, 1):
-            for dj in (-1, 0, 1):
-                if di == 0 and dj == 0:
-                    continue
-                ni, nj = i + di, j + dj
-                if 0 <= ni < nx and 0 <= nj < ny:
-                    drop = (flat[idx] - flat[ni * ny + nj]) / (dx * math.hypot(di, dj))
-                    if drop > best:
-                        best, bi = drop, ni * ny + nj
-        if bi >= 0:
-            acc[bi] += acc[idx]
-    acc = acc.reshape(nx, ny)
-    inc = np.where(acc > threshold_cells, depth * np.minimum(1.0, np.sqrt(acc / (threshold_cells * 100.0))), 0.0)
-    return z - inc
+    if n_channels <= 0:
+        n_channels = max(2, nx // 128)
+    if width <= 0:
+        width = max(2.0, nx / (6.0 * n_channels))
+    j = np.arange(ny, dtype=np.float64)
+    i = np.arange(nx, dtype=np.float64)[:, None]
+    out = z.copy()
+    centres = (np.arange(n_channels) + 0.5) * nx / n_channels
+    for k in range(n_channels):
+        amp = rng.uniform(0.02, 0.08) * nx / n_channels * 4
+        lam = rng.uniform(0.2, 0.6) * ny
+        phase = rng.uniform(0, 2 * np.pi)
+        walk = np.cumsum(rng.normal(0.0, 0.3, ny))
+        # pairs of channels converge towards the outlet (j = 0)
+        target = centres[(k // 2) * 2] if k % 2 else centres[k]
+        blend = np.clip(1 - j / (0.5 * ny), 0, 1)
+        ic = (1 - blend) * centres[k] + blend * 0.5 * (centres[k] + target)
+        ic = ic + amp * np.sin(2 * np.pi * j / lam + phase) + walk
+        dep = depth * (0.4 + 0.6 * (1 - j / ny))
+        out -= dep[None, :] * np.exp(-((i - ic[None, :]) / width) ** 2)
+    return out
 
 
 def planar_rough(nx: int, ny: int, dx: float, slope: float, sigma: float, rng: np.random.Generator) -> np.ndarray:
@@ -316,7 +316,7 @@ def config_c3(seed: int = 3, n: int = 4096, tend: float = 3600.0, carve: bool = 
     rng = np.random.default_rng(seed)
     topo = fractal_topography(n, n, 1.0, 20.0, 0.05, 0.8, rng)
     if carve:
-        topo = carve_channels(topo, 1.0, rng, threshold_cells=max(50, n * n // 20000))
+        topo = carve_channels(topo, 1.0, rng)
     rint = 10.0
     tk = rint * (np.arange(int(tend / rint)) + 0.5)
     peak = 120 * MM_PER_H

@@ -53,6 +53,9 @@ def lib():
     L.swehr_download.argtypes = [C.c_void_p, C.c_int, C.c_int, P]
     L.swehr_stage.argtypes = [C.c_void_p, P, P, C.c_int, C.POINTER(C.c_int)]
     L.swehr_budget.argtypes = [C.c_void_p, P]
+    L.swehr_instance_bytes_for.restype = C.c_int64
+    L.swehr_instance_bytes_for.argtypes = [C.POINTER(_abi.Params), C.c_int, C.c_int]
+    L.swehr_server_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.swehr_instance_bytes.restype = C.c_int64
     L.swehr_instance_bytes.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
     L.swehr_destroy.argtypes = [C.c_void_p]
@@ -105,6 +108,12 @@ def default_params() -> _abi.Params:
 
 def instance_bytes(nx: int, ny: int, nclass: int, precision: int = _abi.FP64) -> int:
     return int(lib().swehr_instance_bytes(nx, ny, nclass, precision))
+
+
+def instance_bytes_for(case, precision: int = _abi.FP64) -> int:
+    """Exact device bytes swehr_create allocates for this case."""
+    p = _abi.params_from_case(case)
+    return int(lib().swehr_instance_bytes_for(C.byref(p), precision, 1 if case.ks_raster is not None else 0))
 
 
 def tile_shapes():
@@ -361,6 +370,11 @@ class Server:
         h = C.c_void_p()
         check("swehr_server_request", lib().swehr_server_request(self._s, int(rid), C.byref(h)))
         return Solver(self.cases[rid], self.precision, _borrow=h)
+
+    def stats(self):
+        used, budget = C.c_int64(0), C.c_int64(0)
+        check("swehr_server_stats", lib().swehr_server_stats(self._s, C.byref(used), C.byref(budget)))
+        return used.value, budget.value
 
     def release(self, rid: int):
         check("swehr_server_release", lib().swehr_server_release(self._s, int(rid)))

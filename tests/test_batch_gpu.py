@@ -95,13 +95,16 @@ def test_ensemble_step_limit_and_resume():
 def test_server_continuous_batching_results_equal_alone():
     rng = np.random.default_rng(31)
     reqs = [config_c5_request(rng, k, sizes=(32, 48, 64), weights=(1, 1, 1), tmin=30.0, tmax=90.0) for k in range(6)]
-    per = max(sv.instance_bytes(c.nx, c.ny, c.nclass, _abi.FP32) for c in reqs)
-    with sv.Server(mem_bytes=int(2.5 * per), max_slots=4, precision=_abi.FP32) as srv:
+    sizes = [sv.instance_bytes_for(c, _abi.FP32) for c in reqs]
+    budget = int(2.5 * max(sizes))
+    with sv.Server(mem_bytes=budget, max_slots=4, precision=_abi.FP32) as srv:
         ids = [srv.submit(c) for c in reqs]
         done = {}
         max_running = 0
         for _ in range(10000):
             running, queued, finished = srv.pump(16)
+            used, _ = srv.stats()
+            assert used <= budget
             max_running = max(max_running, running)
             while True:
                 d = srv.next_done()
@@ -111,8 +114,7 @@ def test_server_continuous_batching_results_equal_alone():
             if len(done) == len(reqs):
                 break
         assert len(done) == len(reqs)
-        sizes = [sv.instance_bytes(c.nx, c.ny, c.nclass, _abi.FP32) for c in reqs]
-        assert max_running <= min(4, int(2.5 * per) // min(sizes))  # slots and memory budget
+        assert 2 <= max_running <= min(4, budget // min(sizes))  # slots and memory budget
         assert max_running < len(reqs)
         for rid, c in zip(ids, reqs):
             r = srv.result(rid)
