@@ -9,8 +9,8 @@ synchronisations.  ``value`` is cell-class-steps/s = cells x size classes x step
 
 Roofline: the fused step kernel's average duration is measured live (HIP events around
 eagerly launched steps); its algorithmic HBM bytes per launch are cells x B_cell with
-B_cell = 2 w (6 + 2 S) + 16 + w bytes (one read and one write of h, hu, hv, z, MAXDEPTH,
-MAXVEL, ch_k, M_k at w bytes, VINF as FP64, one read of the Ks raster).  ``traffic`` is
+B_cell = 2 w (5 + 2 S) + 40 bytes (one read and one write of h, hu, hv, MAXDEPTH, MAXVEL,
+ch_k, M_k at w bytes, of the bed z and VINF as FP64, one read of the FP64 Ks raster).  ``traffic`` is
 the step kernel's measured HBM bytes per launch, read from profiles/traffic.json (made
 with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs scaled by a copy-kernel calibration).
 
@@ -70,7 +70,7 @@ class Hip:
 
 
 def bytes_per_cell(S: int, w: int, ks_raster: bool = True) -> int:
-    return 2 * w * (6 + 2 * S) + 16 + (w if ks_raster else 0)
+    return 2 * w * (5 + 2 * S) + 32 + (8 if ks_raster else 0)
 
 
 def timed_run(hip, solver, steps):

@@ -56,7 +56,7 @@ struct StepArgs {
     int nx, ny, S, py;
     long long plane;
     R *h[2], *hu[2], *hv[2], *ch[2], *dep[2];
-    R *z[3];
+    double *z[3];  // bed elevation, FP64 in both precisions (slopes are differences of it)
     R *M, *totm, *maxvel, *maxdepth;
     double *vinf;
     const double *ks_raster;  // per-cell Ks (pitched like the state) or null
@@ -106,18 +106,19 @@ __device__ __forceinline__ void neumaier(double &s, double &c, double x)
 
 // One-sided at the domain edges exactly as SWEHR.c:1240-1277; P is an LDS plane with
 // row stride ZJ holding z (or the previous step's z) around the cell at (zi, zj).
-template <typename R, int ZJ>
-__device__ __forceinline__ R slope_x(const R *P, int zi, int zj, int gi, int nx, R odx)
+// Computed in FP64 (the bed is FP64 in both precisions).
+template <int ZJ>
+__device__ __forceinline__ double slope_x(const double *P, int zi, int zj, int gi, int nx, double odx)
 {
-    if (gi != 0 && gi != nx - 1) return R(0.5) * odx * (P[(zi + 1) * ZJ + zj] - P[(zi - 1) * ZJ + zj]);
+    if (gi != 0 && gi != nx - 1) return 0.5 * odx * (P[(zi + 1) * ZJ + zj] - P[(zi - 1) * ZJ + zj]);
     if (gi == nx - 1) return odx * (P[zi * ZJ + zj] - P[(zi - 1) * ZJ + zj]);
     return odx * (P[(zi + 1) * ZJ + zj] - P[zi * ZJ + zj]);
 }
 
-template <typename R, int ZJ>
-__device__ __forceinline__ R slope_y(const R *P, int zi, int zj, int gj, int ny, R odx)
+template <int ZJ>
+__device__ __forceinline__ double slope_y(const double *P, int zi, int zj, int gj, int ny, double odx)
 {
-    if (gj != 0 && gj != ny - 1) return R(0.5) * odx * (P[zi * ZJ + zj + 1] - P[zi * ZJ + zj - 1]);
+    if (gj != 0 && gj != ny - 1) return 0.5 * odx * (P[zi * ZJ + zj + 1] - P[zi * ZJ + zj - 1]);
     if (gj == ny - 1) return odx * (P[zi * ZJ + zj] - P[zi * ZJ + zj - 1]);
     return odx * (P[zi * ZJ + zj + 1] - P[zi * ZJ + zj]);
 }
@@ -129,18 +130,19 @@ __device__ __forceinline__ R slope_y(const R *P, int zi, int zj, int gj, int ny,
 // the previous step's flux pass, from the lagged bed zp (read from global memory).
 template <typename R>
 __device__ __forceinline__ R wave_speed_cell(const StepArgs<R> &A, const R *h, const R *hu, const R *hv,
-                                             const R *zp, int i, int j, const InflowState &in)
+                                             const double *zp, int i, int j, const InflowState &in)
 {
     const int nx = A.nx, ny = A.ny, py = A.py;
-    const R odx = R(1.0 / A.dx);
+    const double odx = 1.0 / A.dx;
     const long long c = (long long)i * py + j;
-    R sx, sy;
-    if (i != 0 && i != nx - 1) sx = R(0.5) * odx * (zp[c + py] - zp[c - py]);
-    else if (i == nx - 1) sx = odx * (zp[c] - zp[i > 0 ? c - py : c]);
-    else sx = odx * (zp[nx > 1 ? c + py : c] - zp[c]);
-    if (j != 0 && j != ny - 1) sy = R(0.5) * odx * (zp[c + 1] - zp[c - 1]);
-    else if (j == ny - 1) sy = odx * (zp[c] - zp[j > 0 ? c - 1 : c]);
-    else sy = odx * (zp[ny > 1 ? c + 1 : c] - zp[c]);
+    double sxd, syd;
+    if (i != 0 && i != nx - 1) sxd = 0.5 * odx * (zp[c + py] - zp[c - py]);
+    else if (i == nx - 1) sxd = odx * (zp[c] - zp[i > 0 ? c - py : c]);
+    else sxd = odx * (zp[nx > 1 ? c + py : c] - zp[c]);
+    if (j != 0 && j != ny - 1) syd = 0.5 * odx * (zp[c + 1] - zp[c - 1]);
+    else if (j == ny - 1) syd = odx * (zp[c] - zp[j > 0 ? c - 1 : c]);
+    else syd = odx * (zp[ny > 1 ? c + 1 : c] - zp[c]);
+    const R sx = R(sxd), sy = R(syd);
     const R g = R(kG) * mrsqrt(sx * sx + sy * sy + R(1));
     const R hc = h[c], huc = hu[c], hvc = hv[c];
     R smax = R(0), s;
@@ -230,7 +232,7 @@ struct TileGeom {
     static constexpr int NFX = (TI + 1) * TJ, NFY = TI * (TJ + 1);
     static size_t lds_bytes(int S)
     {
-        return sizeof(double) * NB_TERMS * 16 + sizeof(R) * ((size_t)(5 + S) * EXT + 2 * ZEXT + 3 * NFX + 3 * NFY);
+        return sizeof(double) * (NB_TERMS * 16 + 2 * ZEXT) + sizeof(R) * ((size_t)(5 + S) * EXT + 3 * NFX + 3 * NFY);
     }
 };
 
@@ -262,13 +264,13 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     const int nx = A.nx, ny = A.ny, py = A.py, S = A.S;
     const long long plane = A.plane;
     const R *hin = A.h[p], *huin = A.hu[p], *hvin = A.hv[p], *chin = A.ch[p], *depin = A.dep[p];
-    const R *zc = A.z[p3], *zp = A.z[(p3 + 2) % 3];
+    const double *zc = A.z[p3], *zp = A.z[(p3 + 2) % 3];
 
     double *s_red = reinterpret_cast<double *>(smem_raw);
-    R *s_h = reinterpret_cast<R *>(s_red + NB_TERMS * 16);
+    double *s_z = s_red + NB_TERMS * 16, *s_zp = s_z + ZEXT;
+    R *s_h = reinterpret_cast<R *>(s_zp + ZEXT);
     R *s_u = s_h + EXT, *s_v = s_u + EXT, *s_ts = s_v + EXT, *s_dep = s_ts + EXT;
-    R *s_z = s_dep + EXT, *s_zp = s_z + ZEXT;
-    R *s_fx = s_zp + ZEXT, *s_fy = s_fx + 3 * NFX;
+    R *s_fx = s_dep + EXT, *s_fy = s_fx + 3 * NFX;
     R *s_sed = s_fy + 3 * NFY;
 
     const R hfilm = R(kHfilm);
@@ -332,14 +334,15 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     }
     __syncthreads();
 
-    const R odx = R(1.0 / A.dx);
+    const double odxd = 1.0 / A.dx;
+    const R odx = R(odxd);
     const R g = R(kG);
     // GZ of the previous step's flux pass at global cell (gi, gj) (SWEHR.c:1279, read
     // lagged at 1135 and 1203); gi, gj are inside the domain.
     auto gz_lag = [&](int gi, int gj) -> R {
         const int zi = gi - (i0 - 2), zj = gj - (j0 - 2);
-        const R a = slope_x<R, ZJ>(s_zp, zi, zj, gi, nx, odx);
-        const R bb = slope_y<R, ZJ>(s_zp, zi, zj, gj, ny, odx);
+        const R a = R(slope_x<ZJ>(s_zp, zi, zj, gi, nx, odxd));
+        const R bb = R(slope_y<ZJ>(s_zp, zi, zj, gj, ny, odxd));
         return g * mrsqrt(a * a + bb * bb + R(1));
     };
 
@@ -400,9 +403,9 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         const R h0 = R(A.h0);
 
         // geometry of this step; GX takes SLOPEY of the previous step (SWEHR.c:1239-1280)
-        const R sxn = slope_x<R, ZJ>(s_z, zi, zj, gi, nx, odx);
-        const R syo = slope_y<R, ZJ>(s_zp, zi, zj, gj, ny, odx);
-        const R syn = slope_y<R, ZJ>(s_z, zi, zj, gj, ny, odx);
+        const R sxn = R(slope_x<ZJ>(s_z, zi, zj, gi, nx, odxd));
+        const R syo = R(slope_y<ZJ>(s_zp, zi, zj, gj, ny, odxd));
+        const R syn = R(slope_y<ZJ>(s_z, zi, zj, gj, ny, odxd));
         const R gx = g * sxn * mrsqrt(sxn * sxn + syo * syo + R(1));
         const R sxs = -gx * h;
         const R n2 = sxn * sxn + syn * syn + R(1);
@@ -430,8 +433,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         const R rho0 = R(kRhow * kPhi + kRhos * (1 - kPhi));
         const int du = gi < nx - 1 ? 1 : 0, dd = gi > 0 ? 1 : 0;
         const int ju = gj < ny - 1 ? 1 : 0, jd = gj > 0 ? 1 : 0;
-        const R sx_up = du ? slope_x<R, ZJ>(s_zp, zi + 1, zj, gi + 1, nx, odx) : sxn;
-        const R sx_dn = dd ? slope_x<R, ZJ>(s_z, zi - 1, zj, gi - 1, nx, odx) : sxn;
+        const R sx_up = du ? R(slope_x<ZJ>(s_zp, zi + 1, zj, gi + 1, nx, odxd)) : sxn;
+        const R sx_dn = dd ? R(slope_x<ZJ>(s_z, zi - 1, zj, gi - 1, nx, odxd)) : sxn;
         const R dep = s_dep[e];
         const R cy = mrsqrt(R(1) + syn * syn);
         const R load = rho0 * gz * dep + rho * gz * h;
@@ -445,8 +448,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         if (pgfx * sxn < R(0)) pgfx = mfabs(pgfx);
         bool fail = lhs > (kco + base * kfa + pgfx);
         if (!fail) {
-            const R sy_up = ju ? slope_y<R, ZJ>(s_zp, zi, zj + 1, gj + 1, ny, odx) : syn;
-            const R sy_dn = jd ? slope_y<R, ZJ>(s_z, zi, zj - 1, gj - 1, ny, odx) : syn;
+            const R sy_up = ju ? R(slope_y<ZJ>(s_zp, zi, zj + 1, gj + 1, ny, odxd)) : syn;
+            const R sy_dn = jd ? R(slope_y<ZJ>(s_z, zi, zj - 1, gj - 1, ny, odxd)) : syn;
             R pgfy = pgf0 * (s_dep[e + ju] * mrsqrt(R(1) + sy_up * sy_up) - s_dep[e - jd] * mrsqrt(R(1) + sy_dn * sy_dn));
             if (pgfy * syn > R(0)) pgfy = R(-1) * mfabs(pgfy);
             if (pgfy * syn < R(0)) pgfy = mfabs(pgfy);
@@ -511,7 +514,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             const double vin = A.vinf[c];
             const double zf = vin / (A.thetas - A.theta0);
             const double hd = (double)hn;
-            const double infl = mfmin(hd - kHfilm, dtd * ks * (zf + A.hf + hd) / zf);
+            // the film depth as stored in this precision, so a dry cell stays exactly dry
+            const double infl = mfmin(hd - (double)hfilm, dtd * ks * (zf + A.hf + hd) / zf);
             A.vinf[c] = vin + infl;
             hn = R(hd - infl);
         }
@@ -522,7 +526,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         const R rsp = rhos * (R(1) - R(kPhi));
         const R kbed = R(1) / rsp;
         const R hdt = R(0.5) * dt;
-        R zn_c = s_z[zi * ZJ + zj];
+        double zn_c = s_z[zi * ZJ + zj];
         R totm = R(0), depn = R(0), chtemp = R(0);
         double sb_out = 0;
         for (int k = 0; k < S; k++) {
@@ -566,11 +570,11 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
                 u = decay * u;
                 u = u + dt * (e1 + e3) + red;
                 u = decay * u;
-                zn_c = zn_c - kbed * (u - chtemp);
+                zn_c = zn_c - (double)(kbed * (u - chtemp));
                 m = m + dt * (e1 + e3) - (u - chtemp);
                 bud[B_DET] += (double)(dt * (e1 + e3));
                 if (fail) {
-                    zn_c = zn_c - R(0.99) * m * kbed;
+                    zn_c = zn_c - (double)(R(0.99) * m * kbed);
                     u = u + R(0.99) * m;
                     totm = R(0);
                     m = R(0.01) * m;
@@ -594,7 +598,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             hvn = R(0);
             for (int k = 0; k < S; k++) {
                 const R u = A.ch[q][k * plane + c];
-                zn_c = zn_c + kbed * u;
+                zn_c = zn_c + (double)(kbed * u);
                 A.M[k * plane + c] = A.M[k * plane + c] + u;
                 A.ch[q][k * plane + c] = R(0);
             }

@@ -165,7 +165,7 @@ __global__ void derive_kernel(StepArgs<R> A, int field, int k, double *out)
         case SWEHR_F_VH: v = (double)A.hv[p][c]; break;
         case SWEHR_F_CH: v = (double)A.ch[p][(long long)(k - 1) * A.plane + c]; break;
         case SWEHR_F_M: v = (double)A.M[(long long)(k - 1) * A.plane + c]; break;
-        case SWEHR_F_TOPO: v = (double)A.z[p3][c]; break;
+        case SWEHR_F_TOPO: v = A.z[p3][c]; break;
         case SWEHR_F_VINF: v = A.vinf[c]; break;
         case SWEHR_F_TOTM: v = (double)A.totm[c]; break;
         case SWEHR_F_DEP: v = (double)A.dep[p][c]; break;
@@ -210,7 +210,7 @@ __global__ __launch_bounds__(256) void budget_kernel(StepArgs<R> A, double *out)
             neumaier(s[2], c[2], (double)A.ch[p][k * A.plane + q]);
             neumaier(s[3], c[3], (double)A.M[k * A.plane + q]);
         }
-        neumaier(s[4], c[4], (double)A.z[p3][q]);
+        neumaier(s[4], c[4], A.z[p3][q]);
     }
     for (long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x; b < A.bud_slots;
          b += (long long)gridDim.x * blockDim.x)
@@ -284,7 +284,7 @@ size_t plan(swehr_handle *h, char *base, bool ks_raster)
         o_ch[b] = L.add(pl * S);
         o_dep[b] = L.add(pl);
     }
-    for (int b = 0; b < 3; b++) o_z[b] = L.add(pl);
+    for (int b = 0; b < 3; b++) o_z[b] = L.add(pd);
     const size_t o_M = L.add(pl * S), o_totm = L.add(pl), o_mv = L.add(pl), o_md = L.add(pl);
     const size_t o_vinf = L.add(pd);
     const size_t o_ks = ks_raster ? L.add(pd) : 0;
@@ -304,7 +304,7 @@ size_t plan(swehr_handle *h, char *base, bool ks_raster)
             A.ch[b] = (R *)(base + o_ch[b]);
             A.dep[b] = (R *)(base + o_dep[b]);
         }
-        for (int b = 0; b < 3; b++) A.z[b] = (R *)(base + o_z[b]);
+        for (int b = 0; b < 3; b++) A.z[b] = (double *)(base + o_z[b]);
         A.M = (R *)(base + o_M);
         A.totm = (R *)(base + o_totm);
         A.maxvel = (R *)(base + o_mv);
@@ -550,8 +550,8 @@ extern "C" int swehr_create(const swehr_params *p, const double *topo, const dou
             if (!rc) rc = upload_raster<double>(h, h->a64.z[2], topo);
             if (!rc && ksr) rc = upload_raster<double>(h, const_cast<double *>(h->a64.ks_raster), ks);
         } else {
-            rc = upload_raster<float>(h, h->a32.z[0], topo);
-            if (!rc) rc = upload_raster<float>(h, h->a32.z[2], topo);
+            rc = upload_raster<double>(h, h->a32.z[0], topo);
+            if (!rc) rc = upload_raster<double>(h, h->a32.z[2], topo);
             if (!rc && ksr) rc = upload_raster<double>(h, const_cast<double *>(h->a32.ks_raster), ks);
         }
         if (rc) break;
@@ -583,7 +583,7 @@ extern "C" int swehr_create(const swehr_params *p, const double *topo, const dou
     }
     double s = 0, cc = 0;
     for (long long n = 0; n < (long long)h->nx * h->ny; n++) {
-        const double v = precision == SWEHR_FP64 ? topo[n] : (double)(float)topo[n];
+        const double v = topo[n];
         const double t = s + v;
         if (std::fabs(s) >= std::fabs(v)) cc += (s - t) + v;
         else cc += (v - t) + s;

@@ -93,13 +93,16 @@ def test_ensemble_step_limit_and_resume():
 
 
 def test_server_continuous_batching_results_equal_alone():
+    """Requests admitted as slots and memory free up; each result equals the same case
+    run alone.  Finished requests hold their device memory until released, so the
+    client checks and releases each one as it completes."""
     rng = np.random.default_rng(31)
     reqs = [config_c5_request(rng, k, sizes=(32, 48, 64), weights=(1, 1, 1), tmin=30.0, tmax=90.0) for k in range(6)]
     sizes = [sv.instance_bytes_for(c, _abi.FP32) for c in reqs]
     budget = int(2.5 * max(sizes))
+    checked = 0
     with sv.Server(mem_bytes=budget, max_slots=4, precision=_abi.FP32) as srv:
-        ids = [srv.submit(c) for c in reqs]
-        done = {}
+        ids = {srv.submit(c): c for c in reqs}
         max_running = 0
         for _ in range(10000):
             running, queued, finished = srv.pump(16)
@@ -110,25 +113,26 @@ def test_server_continuous_batching_results_equal_alone():
                 d = srv.next_done()
                 if d is None:
                     break
-                done[d[0]] = d[1]
-            if len(done) == len(reqs):
+                rid, info = d
+                c = ids[rid]
+                r = srv.result(rid)
+                with sv.Solver(c, precision=_abi.FP32) as s:
+                    s.run()
+                    assert s.info()["steps"] == info["steps"]
+                    _same(r, s, c)
+                srv.release(rid)
+                checked += 1
+            if checked == len(reqs):
                 break
-        assert len(done) == len(reqs)
+        assert checked == len(reqs)
         assert 2 <= max_running <= min(4, budget // min(sizes))  # slots and memory budget
         assert max_running < len(reqs)
-        for rid, c in zip(ids, reqs):
-            r = srv.result(rid)
-            with sv.Solver(c, precision=_abi.FP32) as s:
-                s.run()
-                assert s.info()["steps"] == done[rid]["steps"]
-                _same(r, s, c)
-            srv.release(rid)
 
 
 def test_server_rejects_request_larger_than_budget():
     rng = np.random.default_rng(32)
     c = config_c5_request(rng, 0, sizes=(64,), weights=(1,), tmin=30, tmax=40)
-    with sv.Server(mem_bytes=1 << 20, max_slots=2, precision=_abi.FP32) as srv:
+    with sv.Server(mem_bytes=64 << 10, max_slots=2, precision=_abi.FP32) as srv:
         with pytest.raises(sv.SwehrError) as ei:
             srv.submit(c)
         assert ei.value.code == _abi.ERR_BUSY
