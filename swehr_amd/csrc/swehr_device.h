@@ -278,59 +278,153 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     const R inv_rhos = R(1.0 / kRhos);
     const int tid = threadIdx.x;
 
+    // Own-cell values of the update, loaded first so that their latency overlaps the
+    // halo staging below (every load of a phase is issued before the first wait).
+    const int il = tid / TJ, jl = tid - (tid / TJ) * TJ;
+    const int gi = i0 + il, gj = j0 + jl;
+    const bool own = gi < nx && gj < ny;
+    const long long c = (long long)gi * py + gj;
+    R hu_o = R(0), hv_o = R(0), totm_o = R(0), mv_o = R(0), md_o = R(0);
+    double vinf_o = 0, ks_o = A.ks;
+    // ch_k and M_k of the first class (the class loop prefetches one class ahead)
+    R ch_a = R(0), m_a = R(0);
+    if (own) {
+        hu_o = huin[c];
+        hv_o = hvin[c];
+        totm_o = A.totm[c];
+        mv_o = A.maxvel[c];
+        md_o = A.maxdepth[c];
+        vinf_o = A.vinf[c];
+        if (A.ks_raster) ks_o = A.ks_raster[c];
+        ch_a = chin[c];
+        m_a = A.M[c];
+
+    }
+
     // ---- phase 1: ghosts and the 1-cell halo of the conserved state (SWEHR.c:816-1035)
+    constexpr int EIT = (EXT + NT - 1) / NT, ZIT = (ZEXT + NT - 1) / NT;
+    double zc_r[ZIT], zp_r[ZIT];
+#pragma unroll
+    for (int r = 0; r < ZIT; r++) {
+        const int e = tid + r * NT;
+        if (e < ZEXT) {
+            const int zi = e / ZJ, zj = e - (e / ZJ) * ZJ;
+            int ggi = i0 - 2 + zi, ggj = j0 - 2 + zj;
+            ggi = ggi < 0 ? 0 : (ggi >= nx ? nx - 1 : ggi);
+            ggj = ggj < 0 ? 0 : (ggj >= ny ? ny - 1 : ggj);
+            const long long cz = (long long)ggi * py + ggj;
+            zc_r[r] = zc[cz];
+            zp_r[r] = zp[cz];
+        }
+    }
+    // ext cell r of this thread: clamped source cell, and its state
+    long long c_r[EIT];
+    R h_r[EIT], hu_r[EIT], hv_r[EIT], dep_r[EIT];
+#pragma unroll
+    for (int r = 0; r < EIT; r++) {
+        const int e = tid + r * NT;
+        c_r[r] = 0;
+        if (e < EXT) {
+            const int ei = e / EJ, ej = e - (e / EJ) * EJ;
+            const int ggi = i0 - 1 + ei, ggj = j0 - 1 + ej;
+            const int ci = ggi < 0 ? 0 : (ggi >= nx ? nx - 1 : ggi);
+            const int cj = ggj < 0 ? 0 : (ggj >= ny ? ny - 1 : ggj);
+            c_r[r] = (long long)ci * py + cj;
+            h_r[r] = hin[c_r[r]];
+            hu_r[r] = huin[c_r[r]];
+            hv_r[r] = hvin[c_r[r]];
+            dep_r[r] = depin[c_r[r]];
+        }
+    }
+    // suspended mass of the ext cells, up to KC classes per batch of loads
+    constexpr int KC = 8;
+    R ch_r[KC][EIT];
+#pragma unroll
+    for (int kk = 0; kk < KC; kk++)
+#pragma unroll
+        for (int r = 0; r < EIT; r++)
+            if (kk < S && tid + r * NT < EXT) ch_r[kk][r] = chin[kk * plane + c_r[r]];
+
+    // boundary rules and velocities of the ext cells
     const InflowState in = inflow_state(t);
-    for (int e = tid; e < EXT; e += NT) {
-        const int ei = e / EJ, ej = e - (e / EJ) * EJ;
-        const int gi = i0 - 1 + ei, gj = j0 - 1 + ej;
-        const int ci = gi < 0 ? 0 : (gi >= nx ? nx - 1 : gi);
-        const int cj = gj < 0 ? 0 : (gj >= ny ? ny - 1 : gj);
-        const long long c = (long long)ci * py + cj;
-        R h = hin[c], hu = huin[c], hv = hvin[c];
-        if (gi < 0 && A.bndry[0] == 1) hu = -hu;
-        if (gi >= nx && A.bndry[2] == 1) hu = -hu;
-        if (gj < 0 && A.bndry[1] == 1) hv = -hv;
-        const bool inflow_ghost = gj >= ny && A.inflow;
-        if (gj >= ny && !A.inflow && A.bndry[3] == 1) hv = -hv;
-        R ts = R(0), u = R(0), v = R(0);
-        if (inflow_ghost) {
-            h = R(in.h);
-            hu = R(0);
-            hv = R(in.hv);
-            const R sd = R(in.c) / h;
-            for (int k = 0; k < S; k++) s_sed[k * EXT + e] = sd;
+    R ih_r[EIT], ts_r[EIT], sdin_r[EIT];
+    bool wet_r[EIT], ghost_r[EIT];
+#pragma unroll
+    for (int r = 0; r < EIT; r++) {
+        const int e = tid + r * NT;
+        ih_r[r] = R(0);
+        ts_r[r] = R(0);
+        sdin_r[r] = R(0);
+        wet_r[r] = false;
+        ghost_r[r] = false;
+        if (e < EXT) {
+            const int ei = e / EJ, ej = e - (e / EJ) * EJ;
+            const int ggi = i0 - 1 + ei, ggj = j0 - 1 + ej;
+            R h = h_r[r], hu = hu_r[r], hv = hv_r[r];
+            if (ggi < 0 && A.bndry[0] == 1) hu = -hu;
+            if (ggi >= nx && A.bndry[2] == 1) hu = -hu;
+            if (ggj < 0 && A.bndry[1] == 1) hv = -hv;
+            const bool inflow_ghost = ggj >= ny && A.inflow;
+            if (ggj >= ny && !A.inflow && A.bndry[3] == 1) hv = -hv;
+            if (inflow_ghost) {
+                h = R(in.h);
+                hu = R(0);
+                hv = R(in.hv);
+                sdin_r[r] = R(in.c) / h;
+            }
+            R u = R(0), v = R(0);
             if (h > hfilm) {
                 const R ih = R(1) / h;
+                ih_r[r] = ih;
                 u = hu * ih;
                 v = hv * ih;
             }
-        } else if (h > hfilm) {
-            const R ih = R(1) / h;
-            u = hu * ih;
-            v = hv * ih;
-            for (int k = 0; k < S; k++) {
-                R sd = chin[k * plane + c] * ih;
-                ts += sd * inv_rhos;
-                if (sd < R(-1e-5)) sd = R(0);
-                s_sed[k * EXT + e] = sd;
-            }
-        } else {
-            for (int k = 0; k < S; k++) s_sed[k * EXT + e] = R(0);
+            wet_r[r] = h > hfilm;
+            ghost_r[r] = inflow_ghost;
+            s_h[e] = h;
+            s_u[e] = u;
+            s_v[e] = v;
+            s_dep[e] = dep_r[r];
         }
-        s_h[e] = h;
-        s_u[e] = u;
-        s_v[e] = v;
-        s_ts[e] = ts;
-        s_dep[e] = depin[c];
     }
-    for (int e = tid; e < ZEXT; e += NT) {
-        const int zi = e / ZJ, zj = e - (e / ZJ) * ZJ;
-        int gi = i0 - 2 + zi, gj = j0 - 2 + zj;
-        gi = gi < 0 ? 0 : (gi >= nx ? nx - 1 : gi);
-        gj = gj < 0 ? 0 : (gj >= ny ? ny - 1 : gj);
-        const long long c = (long long)gi * py + gj;
-        s_z[e] = zc[c];
-        s_zp[e] = zp[c];
+    for (int k0 = 0; k0 < S; k0 += KC) {
+        if (k0 > 0) {
+#pragma unroll
+            for (int kk = 0; kk < KC; kk++)
+#pragma unroll
+                for (int r = 0; r < EIT; r++)
+                    if (k0 + kk < S && tid + r * NT < EXT) ch_r[kk][r] = chin[(k0 + kk) * plane + c_r[r]];
+        }
+#pragma unroll
+        for (int kk = 0; kk < KC; kk++) {
+            if (k0 + kk >= S) break;
+#pragma unroll
+            for (int r = 0; r < EIT; r++) {
+                const int e = tid + r * NT;
+                if (e < EXT) {
+                    R sd = R(0);
+                    if (ghost_r[r]) {
+                        sd = sdin_r[r];
+                    } else if (wet_r[r]) {
+                        sd = ch_r[kk][r] * ih_r[r];
+                        ts_r[r] += sd * inv_rhos;
+                        if (sd < R(-1e-5)) sd = R(0);
+                    }
+                    s_sed[(k0 + kk) * EXT + e] = sd;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < EIT; r++)
+        if (tid + r * NT < EXT) s_ts[tid + r * NT] = ts_r[r];
+#pragma unroll
+    for (int r = 0; r < ZIT; r++) {
+        const int e = tid + r * NT;
+        if (e < ZEXT) {
+            s_z[e] = zc_r[r];
+            s_zp[e] = zp_r[r];
+        }
     }
     __syncthreads();
 
@@ -347,7 +441,6 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     };
 
     // ---- phase 2: face fluxes of the tile, each computed once
-    const int il = tid / TJ, jl = tid - (tid / TJ) * TJ;
     // x face between ext cells el (left) and el + EJ, with g and cl of the left cell
     auto xface_g = [&](int fi, int fj, R gz, R cl) {
         const int el = fi * EJ + fj + 1, er = el + EJ;
@@ -370,8 +463,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     };
     {
         // the cell's own faces i+1/2 and j+1/2 share its lagged GZ and celerity
-        const int gi = i0 + il, gj = j0 + jl;
-        if (gi < nx && gj < ny) {
+        if (own) {
             const R gz = gz_lag(gi, gj);
             const R cl = msqrt(gz * s_h[(il + 1) * EJ + jl + 1]);
             xface_g(il + 1, jl, gz, cl);
@@ -392,12 +484,10 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
 
     // ---- phase 3: sources and update of the tile's own cells
     double bud[NB_TERMS] = {0, 0, 0, 0, 0};
-    const int gi = i0 + il, gj = j0 + jl;
-    if (gi < nx && gj < ny) {
-        const long long c = (long long)gi * py + gj;
+    if (own) {
         const int e = (il + 1) * EJ + (jl + 1);
         const int zi = il + 2, zj = jl + 2;
-        const R h = s_h[e], hu = huin[c], hv = hvin[c];
+        const R h = s_h[e], hu = hu_o, hv = hv_o;
         const R ts = s_ts[e];
         const R dt = R(dtd), Rr = R(Rd);
         const R h0 = R(A.h0);
@@ -465,7 +555,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
 
         // class-invariant parts of the Hairsine-Rose sources (SWEHR.c:1329-1378);
         // depth normal to the bed h / cos(atan|slope|) = h * msqrt(1 + slope^2)
-        const R totm_old = A.totm[c];
+        const R totm_old = totm_o;
         const R hvert = h * (n2 * rn);
         const bool deep_v = hvert > h0;
         const R ihv = deep_v ? R(1) / hvert : R(0);
@@ -510,8 +600,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
 
         // Green-Ampt infiltration in FP64; negative below the film depth (SWEHR.c:1448-1452)
         {
-            const double ks = A.ks_raster ? A.ks_raster[c] : A.ks;
-            const double vin = A.vinf[c];
+            const double ks = ks_o;
+            const double vin = vinf_o;
             const double zf = vin / (A.thetas - A.theta0);
             const double hd = (double)hn;
             // the film depth as stored in this precision, so a dry cell stays exactly dry
@@ -529,7 +619,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         double zn_c = s_z[zi * ZJ + zj];
         R totm = R(0), depn = R(0), chtemp = R(0);
         double sb_out = 0;
-        for (int k = 0; k < S; k++) {
+        // class k from its suspended mass ch_k and deposited mass m_k
+        auto update_class = [&](int k, R ch_k, R m_k) {
             const R *sk = s_sed + k * EXT;
             const R sc = sk[e];
             const R fkxl = fx0l > R(0) ? fx0l * sk[exl] : fx0l * sc;
@@ -540,8 +631,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             if (gi == nx - 1) sb_out += (double)fkxr;
             if (gj == 0) sb_out -= (double)fkyl;
             if (gj == ny - 1) sb_out += (double)fkyr;
-            R u = chin[k * plane + c] - dx_ * (fkxr - fkxl) - dx_ * (fkyr - fkyl);
-            R m = A.M[k * plane + c];
+            R u = ch_k - dx_ * (fkxr - fkxl) - dx_ * (fkyr - fkyl);
+            R m = m_k;
             // sources of class k from the state before this update (SWEHR.c:1322-1399)
             const R dk = R(A.ds[k]), pk = R(A.pf[k]);
             R e1 = R(0), e2 = R(0), e3 = R(0), e4 = R(0);
@@ -588,6 +679,19 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             depn += m * inv_rhos;
             A.ch[q][k * plane + c] = u;
             A.M[k * plane + c] = m;
+        };
+        // Two classes per trip with two register sets, each refilled for class k + 2
+        // as soon as it is consumed: the loads of a class are in flight for two class
+        // updates and no register copy waits on them.
+        // the next class's loads are issued before this class's update, so they are in
+        // flight for one class update
+        for (int k = 0; k < S; k++) {
+            const R ch_k = ch_a, m_k = m_a;
+            if (k + 1 < S) {
+                ch_a = chin[(k + 1) * plane + c];
+                m_a = A.M[(k + 1) * plane + c];
+            }
+            update_class(k, ch_k, m_k);
         }
 
         // dry-cell clamp (SWEHR.c:1532-1551)
@@ -596,11 +700,17 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             hn = hfilm;
             hun = R(0);
             hvn = R(0);
+            R *chq = A.ch[q];
+            R u_a = chq[c], mm_a = A.M[c];
             for (int k = 0; k < S; k++) {
-                const R u = A.ch[q][k * plane + c];
+                const R u = u_a, mm = mm_a;
+                if (k + 1 < S) {
+                    u_a = chq[(k + 1) * plane + c];
+                    mm_a = A.M[(k + 1) * plane + c];
+                }
                 zn_c = zn_c + (double)(kbed * u);
-                A.M[k * plane + c] = A.M[k * plane + c] + u;
-                A.ch[q][k * plane + c] = R(0);
+                A.M[k * plane + c] = mm + u;
+                chq[k * plane + c] = R(0);
             }
         }
         // implicit friction (SWEHR.c:1553-1565) and maxima (SWEHR.c:1587-1588)
@@ -615,8 +725,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             v2 = hvn * ihn;
         }
         const R speed = msqrt(v1 * v1 + v2 * v2);
-        if (speed > A.maxvel[c]) A.maxvel[c] = speed;
-        if (hn > A.maxdepth[c]) A.maxdepth[c] = hn;
+        if (speed > mv_o) A.maxvel[c] = speed;
+        if (hn > md_o) A.maxdepth[c] = hn;
         A.h[q][c] = hn;
         A.hu[q][c] = hun;
         A.hv[q][c] = hvn;

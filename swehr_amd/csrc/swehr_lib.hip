@@ -416,11 +416,11 @@ int enqueue_step(swehr_handle *h, const StepArgs<R> &A)
 int build_graph(swehr_handle *h)
 {
     if (h->gexec) {
-        hipGraphExecDestroy(h->gexec);
+        (void)hipGraphExecDestroy(h->gexec);
         h->gexec = nullptr;
     }
     if (h->graph) {
-        hipGraphDestroy(h->graph);
+        (void)hipGraphDestroy(h->graph);
         h->graph = nullptr;
     }
     HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
@@ -601,7 +601,7 @@ extern "C" int swehr_configure(swehr_handle *h, int pause_on_print, int steps_pe
     if (steps_per_graph > 0 && steps_per_graph != h->graph_steps) {
         h->graph_steps = steps_per_graph;
         if (h->gexec) {
-            hipGraphExecDestroy(h->gexec);
+            (void)hipGraphExecDestroy(h->gexec);
             h->gexec = nullptr;
         }
     }
@@ -623,7 +623,7 @@ extern "C" int swehr_configure(swehr_handle *h, int pause_on_print, int steps_pe
             h->a32.nblocks = h->nblocks; h->a32.blocks_per_xcd = h->blocks_per_xcd;
         }
         if (h->gexec) {
-            hipGraphExecDestroy(h->gexec);
+            (void)hipGraphExecDestroy(h->gexec);
             h->gexec = nullptr;
         }
     }
