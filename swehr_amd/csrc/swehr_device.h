@@ -45,10 +45,15 @@ struct alignas(64) DevCtrl {
     double t_prev;  // time at the start of the last step (the CFL message prints it)
     double stageinterval, printinterval, printint, rain_sum;
     double rain_comp;
-    unsigned long long smax_bits;
+    // functions of t alone, set with t: the sediment-laden inflow state at j = ny
+    // (SWEHR.c:1019-1033) and the rain rate (SWEHR.c:813-814)
+    double in_h, in_hv, in_c, rain_now;
     long long steps, step_limit;
     int stagecnt, status, done, print_due;
     int pause_on_print, pad[3];
+    // max |wave speed| of the step as bit patterns of non-negative doubles, spread over
+    // 64 slots so that the per-block atomicMax of K1 does not serialise on one address
+    unsigned long long smax_slots[64];
 };
 
 template <typename R>
@@ -75,11 +80,6 @@ struct StepArgs {
     double *saveflow;  // [kFlowCols][stage_cap]
 };
 
-__device__ __forceinline__ bool step_active(const DevCtrl *C)
-{
-    return !C->done && !C->print_due && C->steps < C->step_limit;
-}
-
 template <typename T>
 __device__ __forceinline__ T wave_max(T v)
 {
@@ -88,6 +88,44 @@ __device__ __forceinline__ T wave_max(T v)
         v = w > v ? w : v;
     }
     return v;
+}
+
+__device__ __forceinline__ bool step_active(const DevCtrl *C)
+{
+    return !C->done && !C->print_due && C->steps < C->step_limit;
+}
+
+// Max over the 64 wave-speed slots; every lane of the calling wave gets it (all 64
+// lanes must be active).
+__device__ __forceinline__ double ctrl_smax(const DevCtrl *C)
+{
+    return wave_max(__longlong_as_double((long long)C->smax_slots[threadIdx.x & 63]));
+}
+
+__device__ __forceinline__ InflowState ctrl_inflow(const DevCtrl *C)
+{
+    InflowState in;
+    in.h = C->in_h;
+    in.hv = C->in_hv;
+    in.c = C->in_c;
+    return in;
+}
+
+// Sets the functions of t kept in the control block (one thread).
+template <typename R>
+__device__ __forceinline__ void set_time_functions(DevCtrl *C, const StepArgs<R> &A, double t)
+{
+    const InflowState in = inflow_state(t);
+    C->in_h = in.h;
+    C->in_hv = in.hv;
+    C->in_c = in.c;
+    C->rain_now = rain_rate(A.rain, A.rnum, A.rint, t);
+}
+
+template <typename R>
+__global__ void time_functions_kernel(StepArgs<R> A)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) set_time_functions(A.ctrl, A, A.ctrl->t);
 }
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -172,9 +210,10 @@ __device__ __forceinline__ R wave_speed_cell(const StepArgs<R> &A, const R *h, c
     return smax;
 }
 
-// Block-wide max of v, then one atomicMax on the instance's Smax.
-__device__ __forceinline__ void block_max_to(unsigned long long *dst, double v)
+// Block-wide max of v, then one atomicMax on slot `slot` (0-63) of the instance.
+__device__ __forceinline__ void block_max_to(unsigned long long *slots, int slot, double v)
 {
+    unsigned long long *dst = slots + (slot & 63);
     __shared__ double red[16];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     v = wave_max(v);
@@ -186,6 +225,9 @@ __device__ __forceinline__ void block_max_to(unsigned long long *dst, double v)
     }
 }
 
+// One thread per cell on 4 x 64 tiles (a wave reads one 64-cell row segment of each
+// field); tiles are dealt to XCDs in contiguous runs so that the i+1 row a tile reads
+// is in the same XCD's L2.  The grid is the tile count rounded up to a multiple of 8.
 template <typename R>
 __global__ __launch_bounds__(256) void wave_speed_kernel(StepArgs<R> A)
 {
@@ -193,16 +235,17 @@ __global__ __launch_bounds__(256) void wave_speed_kernel(StepArgs<R> A)
     if (!step_active(C)) return;
     const long long steps = C->steps;
     const int p = (int)(steps & 1), p3 = (int)(steps % 3);
-    const InflowState in = inflow_state(C->t);
-    R smax = R(0);
-    const long long n = (long long)A.nx * A.ny;
-    for (long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x; cell < n;
-         cell += (long long)gridDim.x * blockDim.x) {
-        const int i = (int)(cell / A.ny), j = (int)(cell % A.ny);
-        const R s = wave_speed_cell(A, A.h[p], A.hu[p], A.hv[p], A.z[(p3 + 2) % 3], i, j, in);
-        smax = s > smax ? s : smax;
+    const InflowState in = ctrl_inflow(C);
+    const int tiles_j = (A.ny + 63) >> 6, ntiles = ((A.nx + 3) >> 2) * tiles_j;
+    const int b = blockIdx.x, per = (int)(gridDim.x >> 3);
+    const int tile = (b & 7) * per + (b >> 3);
+    R s = R(0);
+    if (tile < ntiles) {
+        const int ti = tile / tiles_j, tj = tile - ti * tiles_j;
+        const int i = ti * 4 + (int)(threadIdx.x >> 6), j = tj * 64 + (int)(threadIdx.x & 63);
+        if (i < A.nx && j < A.ny) s = wave_speed_cell(A, A.h[p], A.hu[p], A.hv[p], A.z[(p3 + 2) % 3], i, j, in);
     }
-    block_max_to(&C->smax_bits, (double)smax);
+    block_max_to(C->smax_slots, b, (double)s);
 }
 
 // Wave speeds of one TI x TJ tile (ensemble path: one block per tile).
@@ -213,13 +256,13 @@ __device__ __forceinline__ void wave_speed_tile(const StepArgs<R> &A, int tile)
     if (!step_active(C)) return;
     const long long steps = C->steps;
     const int p = (int)(steps & 1), p3 = (int)(steps % 3);
-    const InflowState in = inflow_state(C->t);
+    const InflowState in = ctrl_inflow(C);
     const int ti = tile / A.tiles_j, tj = tile - (tile / A.tiles_j) * A.tiles_j;
     const int il = threadIdx.x / TJ, jl = threadIdx.x - (threadIdx.x / TJ) * TJ;
     const int i = ti * TI + il, j = tj * TJ + jl;
     R s = R(0);
     if (i < A.nx && j < A.ny) s = wave_speed_cell(A, A.h[p], A.hu[p], A.hv[p], A.z[(p3 + 2) % 3], i, j, in);
-    block_max_to(&C->smax_bits, (double)s);
+    block_max_to(C->smax_slots, tile, (double)s);
 }
 
 // --------------------------------------------------------------------------- K2
@@ -257,9 +300,9 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
 
     const long long steps = C->steps;
     const double t = C->t;
-    const double smax = __longlong_as_double((long long)C->smax_bits);
+    const double smax = ctrl_smax(C);
     const double dtd = step_dt(smax, A.cstable, A.dx, A.maxstep, t, A.tend);
-    const double Rd = rain_rate(A.rain, A.rnum, A.rint, t);
+    const double Rd = C->rain_now;
     const int p = (int)(steps & 1), q = p ^ 1, p3 = (int)(steps % 3);
     const int nx = A.nx, ny = A.ny, py = A.py, S = A.S;
     const long long plane = A.plane;
@@ -346,7 +389,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             if (kk < S && tid + r * NT < EXT) ch_r[kk][r] = chin[kk * plane + c_r[r]];
 
     // boundary rules and velocities of the ext cells
-    const InflowState in = inflow_state(t);
+    const InflowState in = ctrl_inflow(C);
     R ih_r[EIT], ts_r[EIT], sdin_r[EIT];
     bool wet_r[EIT], ghost_r[EIT];
 #pragma unroll
@@ -576,7 +619,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         if (active) {
             const R qq = hu * hu + hv * hv;
             const R qm = msqrt(qq);
-            const R sf = manning2_over_hpow(h, rough, R(3.333)) * qq;
+            const R sf = manning2_over_hpow<3>(h, rough) * qq;
             omega = rho * gz * sf * qm;
             const R oneH = R(1) - H;
             f1 = oneH * R(A.a) * Rr * shield;
@@ -716,7 +759,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         // implicit friction (SWEHR.c:1553-1565) and maxima (SWEHR.c:1587-1588)
         R v1 = R(0), v2 = R(0);
         if (hn > hfilm) {
-            const R sf2 = manning2_over_hpow(hn, rough, R(2.333)) * gz * msqrt(hun * hun + hvn * hvn);
+            const R sf2 = manning2_over_hpow<2>(hn, rough) * gz * msqrt(hun * hun + hvn * hvn);
             const R den = R(1) / (R(1) + dt * sf2);
             hun = hun * den;
             hvn = hvn * den;
@@ -810,10 +853,10 @@ __device__ __forceinline__ void control_body(const StepArgs<R> &A)
     __shared__ int s_idx[256];
     const long long steps = C->steps;
     const double t = C->t;
-    const double smax = __longlong_as_double((long long)C->smax_bits);
+    const double smax = ctrl_smax(C);
     const double dt = step_dt(smax, A.cstable, A.dx, A.maxstep, t, A.tend);
     const double courant = (smax)*dt * (1.0 / A.dx);
-    const double Rr = rain_rate(A.rain, A.rnum, A.rint, t);
+    const double Rr = C->rain_now;
     const bool cfl = courant > 1;
     const double tnew = (cfl ? 2 * A.tend : t) + dt;
     const double stageinterval = C->stageinterval;
@@ -823,51 +866,66 @@ __device__ __forceinline__ void control_body(const StepArgs<R> &A)
     if (tnew > stageinterval) {
         // outlet: deepest cell of row j = 1, first one on ties (SWEHR.c:1596-1604)
         double best = 0;
-        int idx = 0;
+        int idx = 0x7fffffff;
         for (int i = threadIdx.x; i < A.nx; i += blockDim.x) {
             const double hv = (double)A.h[q][(long long)i * A.py];
             if (hv > best) { best = hv; idx = i; }
         }
-        s_best[threadIdx.x] = best;
-        s_idx[threadIdx.x] = best > 0 ? idx : 0x7fffffff;
+        // block arg-max: larger depth wins, the lower index on equal depths
+        for (int o = 32; o > 0; o >>= 1) {
+            const double ob = __shfl_xor(best, o, 64);
+            const int oi = __shfl_xor(idx, o, 64);
+            if (ob > best || (ob == best && oi < idx)) { best = ob; idx = oi; }
+        }
+        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+        if (lane == 0) {
+            s_best[wid] = best;
+            s_idx[wid] = idx;
+        }
         __syncthreads();
         if (threadIdx.x == 0) {
-            double bb = 0;
-            int bi = 0x7fffffff;
-            for (int k = 0; k < (int)blockDim.x; k++)
-                if (s_best[k] > bb || (s_best[k] == bb && bb > 0 && s_idx[k] < bi)) { bb = s_best[k]; bi = s_idx[k]; }
-            const int outlet = bb > 0 ? bi : 0;
-            const int r = C->stagecnt;
-            if (r >= cap) {
-                C->status = 6;  // SWEHR_ERR_STAGE_FULL
-            } else {
-                const long long c = (long long)outlet * A.py;
+            for (int w = 1; w < (int)(blockDim.x >> 6); w++)
+                if (s_best[w] > best || (s_best[w] == best && s_idx[w] < idx)) { best = s_best[w]; idx = s_idx[w]; }
+            s_idx[0] = best > 0 ? idx : 0;
+        }
+        __syncthreads();
+        const int outlet = s_idx[0];
+        const int r = C->stagecnt;
+        // one thread per stage column group: outlet state, its classes, the six probes
+        // at (i = 2, j = 1, 10, ..., 50) (SWEHR.c:1605-1644)
+        const int tid = threadIdx.x;
+        if (r < cap) {
+            double *st = A.stage;
+            const long long co = (long long)outlet * A.py;
+            if (tid == 0) {
                 double h, v1, v2, ts;
-                derived(A, q, c, h, v1, v2, ts);
-                double *st = A.stage;
+                derived(A, q, co, h, v1, v2, ts);
                 st[0 * cap + r] = tnew;
                 st[1 * cap + r] = h;
                 st[2 * cap + r] = sqrt(v1 * v1 + v2 * v2);
                 st[3 * cap + r] = ts;
-                for (int k = 0; k < 10; k++) {
-                    st[(4 + k) * cap + r] = k < A.S ? (double)A.M[k * A.plane + c] : 0.0;
-                    st[(14 + k) * cap + r] = k < A.S ? (double)A.ch[q][k * A.plane + c] : 0.0;
+            } else if (tid >= 1 && tid <= 10) {
+                const int k = tid - 1;
+                st[(4 + k) * cap + r] = k < A.S ? (double)A.M[k * A.plane + co] : 0.0;
+                st[(14 + k) * cap + r] = k < A.S ? (double)A.ch[q][k * A.plane + co] : 0.0;
+            } else if (tid >= 11 && tid <= 16) {
+                const int n = tid - 11;
+                const int yp = n == 0 ? 1 : 10 * n;
+                double d = 0, sp = 0, tt = 0;
+                if (A.nx >= 2 && yp <= A.ny) {
+                    double hh, w1, w2;
+                    derived(A, q, (long long)1 * A.py + (yp - 1), hh, w1, w2, tt);
+                    d = hh;
+                    sp = sqrt(w1 * w1 + w2 * w2);
                 }
-                const int yp[6] = {1, 10, 20, 30, 40, 50};
-                for (int n = 0; n < 6; n++) {
-                    double d = 0, sp = 0, tt = 0;
-                    if (A.nx >= 2 && yp[n] <= A.ny) {
-                        double hh, w1, w2;
-                        derived(A, q, (long long)1 * A.py + (yp[n] - 1), hh, w1, w2, tt);
-                        d = hh;
-                        sp = sqrt(w1 * w1 + w2 * w2);
-                    }
-                    A.saveflow[(3 * n + 0) * cap + r] = d;
-                    A.saveflow[(3 * n + 1) * cap + r] = sp;
-                    A.saveflow[(3 * n + 2) * cap + r] = tt;
-                }
-                C->stagecnt = r + 1;
+                A.saveflow[(3 * n + 0) * cap + r] = d;
+                A.saveflow[(3 * n + 1) * cap + r] = sp;
+                A.saveflow[(3 * n + 2) * cap + r] = tt;
             }
+        }
+        if (tid == 0) {
+            if (r >= cap) C->status = 6;  // SWEHR_ERR_STAGE_FULL
+            else C->stagecnt = r + 1;
             C->stageinterval = stageinterval + 1;
         }
     }
@@ -885,13 +943,15 @@ __device__ __forceinline__ void control_body(const StepArgs<R> &A)
         C->rain_comp = rc;
         C->t_prev = t;
         C->t = tnew;
+        set_time_functions(C, A, tnew);
         C->dt = dt;
         C->courant = courant;
         C->rain = Rr;
         C->steps = steps + 1;
-        C->smax_bits = 0ull;
         if (!(tnew < A.tend) || C->status != 0) C->done = 1;
     }
+    __syncthreads();  // every wave has read the slots
+    if (threadIdx.x < 64) C->smax_slots[threadIdx.x] = 0ull;
 }
 
 template <typename R>

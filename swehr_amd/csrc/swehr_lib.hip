@@ -332,7 +332,7 @@ void geometry(swehr_handle *h, int tile)
     h->nblocks = h->tiles_i * h->tiles_j;
     h->blocks_per_xcd = (h->nblocks + 7) / 8;
     const long long n = (long long)h->nx * h->ny;
-    h->k1_blocks = (int)std::min<long long>((n + 255) / 256, 4096);
+    h->k1_blocks = (int)(8 * ((((h->nx + 3) / 4) * ((h->ny + 63) / 64) + 7) / 8));  // 4 x 64 tiles
     h->budget_blocks = (int)std::min<long long>((n + 255) / 256, 1024);
     h->lds = h->precision == SWEHR_FP64 ? tile_lds<double>(tile, h->S) : tile_lds<float>(tile, h->S);
     int slots = 0;
@@ -574,6 +574,10 @@ extern "C" int swehr_create(const swehr_params *p, const double *topo, const dou
             rc = SWEHR_ERR_HIP;
             break;
         }
+        if (precision == SWEHR_FP64)
+            hipLaunchKernelGGL(time_functions_kernel<double>, dim3(1), dim3(64), 0, h->stream, h->a64);
+        else
+            hipLaunchKernelGGL(time_functions_kernel<float>, dim3(1), dim3(64), 0, h->stream, h->a32);
         if (hipStreamSynchronize(h->stream) != hipSuccess) { rc = SWEHR_ERR_HIP; break; }
     } while (0);
     if (rc) {

@@ -87,26 +87,38 @@ __device__ __forceinline__ R face_flux_v(R hl, R hr, R ul, R ur, R vl, R vr, R c
     return s;
 }
 
-// Natural log and exp used for the friction powers: the library versions in FP64,
-// the hardware-based fast versions in FP32.
-__device__ __forceinline__ double fast_log(double x) { return log(x); }
-__device__ __forceinline__ float fast_log(float x) { return __logf(x); }
-__device__ __forceinline__ double fast_exp(double x) { return exp(x); }
-__device__ __forceinline__ float fast_exp(float x) { return __expf(x); }
-
-// ln(0.003): log of the critical depth of the depth-dependent Manning law (SWEHR.c:490)
+// ln(0.003) and log2(0.003): the critical depth of the depth-dependent Manning law
+// (SWEHR.c:490)
 constexpr double kLnHcFriction = -5.809142990314028;
+constexpr double kLog2HcFriction = -8.380821783940931;
 
 // n^2 / h^p for Manning's n of SWEHR.c:1356-1357 (n0 (h/0.003)^-0.33 below 0.003 m,
-// n0 above): n0^2 exp(-p ln h - 0.66 (ln h - ln 0.003)) or n0^2 exp(-p ln h).  One log
-// and one exp replace pow(h/0.003, -0.33) and pow(h, p) (SWEHR.c:1360, 1559).
-template <typename R>
-__device__ __forceinline__ R manning2_over_hpow(R h, R n0, R p)
+// n0 above), p = PI + 0.333 (3.333 at SWEHR.c:1360, 2.333 at SWEHR.c:1559).
+// FP64: n0^2 exp(-p ln h [- 0.66 (ln h - ln 0.003)]), one log and one exp in place of
+// pow(h/0.003, -0.33) and pow(h, p).
+// FP32: n0^2 h^-PI 2^(-0.333 log2 h [- 0.66 (log2 h - log2 0.003)]).  Splitting off the
+// integer power keeps the argument of the exponential small (|y| < 12 for h >= 1e-5),
+// so its rounding costs ~1e-7 relative instead of the ~3e-6 of exp(-3.333 ln h) at
+// h = 1e-3; the hardware log2/exp2 (1 ulp) and reciprocal are then accurate enough.
+template <int PI, typename R>
+__device__ __forceinline__ R manning2_over_hpow(R h, R n0)
 {
-    const R lh = fast_log(h);
-    R ex = -p * lh;
-    if (h <= R(kHcFriction)) ex += R(2 * kDepthExp) * (lh - R(kLnHcFriction));
-    return n0 * n0 * fast_exp(ex);
+    if constexpr (sizeof(R) == sizeof(double)) {
+        const double p = PI == 3 ? 3.333 : 2.333;
+        const double lh = log(h);
+        double ex = -p * lh;
+        if (h <= kHcFriction) ex += 2 * kDepthExp * (lh - kLnHcFriction);
+        return n0 * n0 * exp(ex);
+    } else {
+        static_assert(PI == 2 || PI == 3, "friction powers 2.333 and 3.333");
+        const float l2 = __builtin_amdgcn_logf(h);
+        float y = -0.333f * l2;
+        if (h <= float(kHcFriction)) y += float(2 * kDepthExp) * (l2 - float(kLog2HcFriction));
+        const float ih = __builtin_amdgcn_rcpf(h);
+        float ihp = ih * ih;
+        if (PI == 3) ihp = ihp * ih;
+        return n0 * n0 * ihp * __builtin_amdgcn_exp2f(y);
+    }
 }
 
 // Wave-speed bound of the same face without the fluxes (SWEHR.c:333-376, 404-408).
