@@ -20,6 +20,7 @@ of steps from the initial state (the GPU is also timed on that window for refere
 from __future__ import annotations
 
 import argparse
+import copy
 import ctypes as C
 import json
 import os
@@ -153,6 +154,15 @@ def bench_precision(hip, grid: int, t_check: float, window: int):
             S_fields = {"depth": s.field("depth"), "speed": s.field("speed"), "stage": st}
             S_fields["sed"] = [s.field("sed", k + 1) for k in range(S)]
             res[name] = (S_fields, s.info(), s.budget())
+    # sensitivity of the FP64 solution itself: the same run with the topography
+    # perturbed by 1e-12 relative (discrete events in the step amplify any change)
+    import numpy as np
+
+    pert = copy.copy(case)
+    pert.topo = case.topo * (1 + 1e-12 * np.random.default_rng(7).standard_normal(case.topo.shape))
+    with sv.Solver(pert, precision=_abi.FP64) as s:
+        s.run()
+        floor = {"depth": s.field("depth"), "speed": s.field("speed")}
     a, b = res["fp32"][0], res["fp64"][0]
     n = min(a["stage"].shape[1], b["stage"].shape[1])
     out["fp32_vs_fp64"] = {
@@ -162,6 +172,10 @@ def bench_precision(hip, grid: int, t_check: float, window: int):
         "rel_l2_outlet_hydrograph": rel_l2(a["stage"][1, :n], b["stage"][1, :n]),
         "rel_l2_outlet_sedigraph": rel_l2(a["stage"][3, :n], b["stage"][3, :n]),
         "steps": [res["fp32"][1]["steps"], res["fp64"][1]["steps"]]}
+    out["fp64_sensitivity"] = {
+        "perturbation": "topography x (1 + 1e-12 N(0,1)), FP64",
+        "rel_l2_depth": rel_l2(floor["depth"], b["depth"]),
+        "rel_l2_speed": rel_l2(floor["speed"], b["speed"])}
     for name in ("fp64", "fp32"):
         we, se = budget_errors(res[name][2])
         out["mass_balance_rel_%s" % name] = {"water": we, "sediment": se}

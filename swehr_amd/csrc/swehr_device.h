@@ -69,13 +69,13 @@ struct StepArgs {
     double ks, dx, h0, mtstar0, a, ad, F, J, roughness, theta0, thetas, hf, cstable, maxstep, tend, rint, vinf0;
     int rnum, inflow, bndry[4];
     int tiles_i, tiles_j, nblocks, blocks_per_xcd;
-    int bud_slots;  // budget slots allocated (>= tile count of every tile shape)
+    int bud_slots;  // budget slots allocated: one per wave of every tile, any tile shape
     int stage_cap;  // rows of the stage / saveflow series
     double ds[kMaxClasses], pf[kMaxClasses];
     // per-class settling constants a_k = 13.95 nu / d_k and a_k^2 (SWEHR.c:435)
     double ak[kMaxClasses], ak2[kMaxClasses];
     DevCtrl *ctrl;
-    double *bud;       // [bud_slots][NB_TERMS][2]
+    double *bud;       // [bud_slots][NB_TERMS][3]: running sum, compensation, last step's partial
     double *stage;     // [kStageCols][stage_cap]
     double *saveflow;  // [kFlowCols][stage_cap]
 };
@@ -90,19 +90,30 @@ __device__ __forceinline__ T wave_max(T v)
     return v;
 }
 
-__device__ __forceinline__ bool step_active(const DevCtrl *C)
+// The pointers in an argument block are device (global) memory.  Typing them so lets
+// the compiler use global (and scalar) instead of flat instructions where the block was
+// read from memory rather than passed as a kernel argument (the ensemble kernels).
+template <typename T>
+using glob = T __attribute__((address_space(1)));
+template <typename T>
+__device__ __forceinline__ glob<T> *gmem(T *p)
+{
+    return (glob<T> *)p;
+}
+
+__device__ __forceinline__ bool step_active(const glob<DevCtrl> *C)
 {
     return !C->done && !C->print_due && C->steps < C->step_limit;
 }
 
 // Max over the 64 wave-speed slots; every lane of the calling wave gets it (all 64
 // lanes must be active).
-__device__ __forceinline__ double ctrl_smax(const DevCtrl *C)
+__device__ __forceinline__ double ctrl_smax(const glob<DevCtrl> *C)
 {
     return wave_max(__longlong_as_double((long long)C->smax_slots[threadIdx.x & 63]));
 }
 
-__device__ __forceinline__ InflowState ctrl_inflow(const DevCtrl *C)
+__device__ __forceinline__ InflowState ctrl_inflow(const glob<DevCtrl> *C)
 {
     InflowState in;
     in.h = C->in_h;
@@ -113,7 +124,7 @@ __device__ __forceinline__ InflowState ctrl_inflow(const DevCtrl *C)
 
 // Sets the functions of t kept in the control block (one thread).
 template <typename R>
-__device__ __forceinline__ void set_time_functions(DevCtrl *C, const StepArgs<R> &A, double t)
+__device__ __forceinline__ void set_time_functions(glob<DevCtrl> *C, const StepArgs<R> &A, double t)
 {
     const InflowState in = inflow_state(t);
     C->in_h = in.h;
@@ -125,7 +136,7 @@ __device__ __forceinline__ void set_time_functions(DevCtrl *C, const StepArgs<R>
 template <typename R>
 __global__ void time_functions_kernel(StepArgs<R> A)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) set_time_functions(A.ctrl, A, A.ctrl->t);
+    if (threadIdx.x == 0 && blockIdx.x == 0) set_time_functions(gmem(A.ctrl), A, A.ctrl->t);
 }
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -161,14 +172,26 @@ __device__ __forceinline__ double slope_y(const double *P, int zi, int zj, int g
     return odx * (P[zi * ZJ + zj + 1] - P[zi * ZJ + zj]);
 }
 
+// Budget slot term p = {running sum, compensation, pending partial}: folds the pending
+// partial into the sum (Neumaier) and clears it.
+__device__ __forceinline__ void fold_budget(glob<double> *p)
+{
+    double s = p[0], c = p[1];
+    neumaier(s, c, p[2]);
+    p[0] = s;
+    p[1] = c;
+    p[2] = 0.0;
+}
+
 // --------------------------------------------------------------------------- K1
 
 // Wave-speed bound of the faces a cell owns: i+1/2 and j+1/2, plus 1/2 at i = 0 or
 // j = 0.  Ghost states follow the boundary rules of SWEHR.c:816-1035; GZ is that of
 // the previous step's flux pass, from the lagged bed zp (read from global memory).
 template <typename R>
-__device__ __forceinline__ R wave_speed_cell(const StepArgs<R> &A, const R *h, const R *hu, const R *hv,
-                                             const double *zp, int i, int j, const InflowState &in)
+__device__ __forceinline__ R wave_speed_cell(const StepArgs<R> &A, const glob<R> *h, const glob<R> *hu,
+                                             const glob<R> *hv, const glob<double> *zp, int i, int j,
+                                             const InflowState &in)
 {
     const int nx = A.nx, ny = A.ny, py = A.py;
     const double odx = 1.0 / A.dx;
@@ -231,7 +254,7 @@ __device__ __forceinline__ void block_max_to(unsigned long long *slots, int slot
 template <typename R>
 __global__ __launch_bounds__(256) void wave_speed_kernel(StepArgs<R> A)
 {
-    DevCtrl *C = A.ctrl;
+    glob<DevCtrl> *C = gmem(A.ctrl);
     if (!step_active(C)) return;
     const long long steps = C->steps;
     const int p = (int)(steps & 1), p3 = (int)(steps % 3);
@@ -243,16 +266,21 @@ __global__ __launch_bounds__(256) void wave_speed_kernel(StepArgs<R> A)
     if (tile < ntiles) {
         const int ti = tile / tiles_j, tj = tile - ti * tiles_j;
         const int i = ti * 4 + (int)(threadIdx.x >> 6), j = tj * 64 + (int)(threadIdx.x & 63);
-        if (i < A.nx && j < A.ny) s = wave_speed_cell(A, A.h[p], A.hu[p], A.hv[p], A.z[(p3 + 2) % 3], i, j, in);
+        if (i < A.nx && j < A.ny)
+            s = wave_speed_cell(A, gmem(A.h[p]), gmem(A.hu[p]), gmem(A.hv[p]), gmem(A.z[(p3 + 2) % 3]), i, j, in);
     }
-    block_max_to(C->smax_slots, b, (double)s);
+    // the step kernel's budget partials of the previous step
+    for (long long g = (long long)b * blockDim.x + threadIdx.x; g < (long long)A.bud_slots * NB_TERMS;
+         g += (long long)gridDim.x * blockDim.x)
+        fold_budget(gmem(A.bud) + 3 * g);
+    block_max_to((unsigned long long *)C->smax_slots, b, (double)s);
 }
 
 // Wave speeds of one TI x TJ tile (ensemble path: one block per tile).
 template <typename R, int TI, int TJ>
 __device__ __forceinline__ void wave_speed_tile(const StepArgs<R> &A, int tile)
 {
-    DevCtrl *C = A.ctrl;
+    glob<DevCtrl> *C = gmem(A.ctrl);
     if (!step_active(C)) return;
     const long long steps = C->steps;
     const int p = (int)(steps & 1), p3 = (int)(steps % 3);
@@ -261,11 +289,20 @@ __device__ __forceinline__ void wave_speed_tile(const StepArgs<R> &A, int tile)
     const int il = threadIdx.x / TJ, jl = threadIdx.x - (threadIdx.x / TJ) * TJ;
     const int i = ti * TI + il, j = tj * TJ + jl;
     R s = R(0);
-    if (i < A.nx && j < A.ny) s = wave_speed_cell(A, A.h[p], A.hu[p], A.hv[p], A.z[(p3 + 2) % 3], i, j, in);
-    block_max_to(C->smax_slots, tile, (double)s);
+    if (i < A.nx && j < A.ny)
+        s = wave_speed_cell(A, gmem(A.h[p]), gmem(A.hu[p]), gmem(A.hv[p]), gmem(A.z[(p3 + 2) % 3]), i, j, in);
+    // the step kernel's budget partials of this tile's slots (previous step)
+    if ((int)threadIdx.x < (TI * TJ / 64) * NB_TERMS)
+        fold_budget(gmem(A.bud) + 3 * ((size_t)tile * (TI * TJ / 64) * NB_TERMS + threadIdx.x));
+    block_max_to((unsigned long long *)C->smax_slots, tile, (double)s);
 }
 
 // --------------------------------------------------------------------------- K2
+
+// Waves per SIMD the step kernels are compiled for (VGPR budget 512 / waves): the
+// LDS of a tile admits 3 blocks per CU in FP64 and 5 in FP32 at the default tile.
+template <typename R>
+constexpr int kStepWaves = sizeof(R) == 4 ? 5 : 3;
 
 template <typename R, int TI, int TJ>
 struct TileGeom {
@@ -275,7 +312,7 @@ struct TileGeom {
     static constexpr int NFX = (TI + 1) * TJ, NFY = TI * (TJ + 1);
     static size_t lds_bytes(int S)
     {
-        return sizeof(double) * (NB_TERMS * 16 + 2 * ZEXT) + sizeof(R) * ((size_t)(5 + S) * EXT + 3 * NFX + 3 * NFY);
+        return sizeof(double) * 2 * ZEXT + sizeof(R) * ((size_t)(5 + S) * EXT + 3 * NFX + 3 * NFY);
     }
 };
 
@@ -292,7 +329,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     using T = TileGeom<R, TI, TJ>;
     constexpr int NT = T::NT, EJ = T::EJ, EXT = T::EXT, ZJ = T::ZJ, ZEXT = T::ZEXT;
     constexpr int NFX = T::NFX, NFY = T::NFY;
-    DevCtrl *C = A.ctrl;
+    glob<DevCtrl> *C = gmem(A.ctrl);
     if (!step_active(C)) return;
 
     const int ti = tile / A.tiles_j, tj = tile - (tile / A.tiles_j) * A.tiles_j;
@@ -306,11 +343,17 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     const int p = (int)(steps & 1), q = p ^ 1, p3 = (int)(steps % 3);
     const int nx = A.nx, ny = A.ny, py = A.py, S = A.S;
     const long long plane = A.plane;
-    const R *hin = A.h[p], *huin = A.hu[p], *hvin = A.hv[p], *chin = A.ch[p], *depin = A.dep[p];
-    const double *zc = A.z[p3], *zp = A.z[(p3 + 2) % 3];
+    const glob<R> *hin = gmem(A.h[p]), *huin = gmem(A.hu[p]), *hvin = gmem(A.hv[p]), *chin = gmem(A.ch[p]);
+    const glob<R> *depin = gmem(A.dep[p]);
+    const glob<double> *zc = gmem(A.z[p3]), *zp = gmem(A.z[(p3 + 2) % 3]);
+    glob<R> *const hout = gmem(A.h[q]), *const huout = gmem(A.hu[q]), *const hvout = gmem(A.hv[q]);
+    glob<R> *const chout = gmem(A.ch[q]), *const depout = gmem(A.dep[q]);
+    glob<double> *const zout = gmem(A.z[(p3 + 1) % 3]);
+    glob<R> *const Mp = gmem(A.M), *const totmp = gmem(A.totm), *const maxvelp = gmem(A.maxvel);
+    glob<R> *const maxdepthp = gmem(A.maxdepth);
+    glob<double> *const vinfp = gmem(A.vinf);
 
-    double *s_red = reinterpret_cast<double *>(smem_raw);
-    double *s_z = s_red + NB_TERMS * 16, *s_zp = s_z + ZEXT;
+    double *s_z = reinterpret_cast<double *>(smem_raw), *s_zp = s_z + ZEXT;
     R *s_h = reinterpret_cast<R *>(s_zp + ZEXT);
     R *s_u = s_h + EXT, *s_v = s_u + EXT, *s_ts = s_v + EXT, *s_dep = s_ts + EXT;
     R *s_fx = s_dep + EXT, *s_fy = s_fx + 3 * NFX;
@@ -329,18 +372,19 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     const long long c = (long long)gi * py + gj;
     R hu_o = R(0), hv_o = R(0), totm_o = R(0), mv_o = R(0), md_o = R(0);
     double vinf_o = 0, ks_o = A.ks;
+    const int lane = tid & 63, wid = tid >> 6;
     // ch_k and M_k of the first class (the class loop prefetches one class ahead)
     R ch_a = R(0), m_a = R(0);
     if (own) {
         hu_o = huin[c];
         hv_o = hvin[c];
-        totm_o = A.totm[c];
-        mv_o = A.maxvel[c];
-        md_o = A.maxdepth[c];
-        vinf_o = A.vinf[c];
-        if (A.ks_raster) ks_o = A.ks_raster[c];
+        totm_o = totmp[c];
+        mv_o = maxvelp[c];
+        md_o = maxdepthp[c];
+        vinf_o = vinfp[c];
+        if (A.ks_raster) ks_o = gmem(A.ks_raster)[c];
         ch_a = chin[c];
-        m_a = A.M[c];
+        m_a = Mp[c];
 
     }
 
@@ -380,7 +424,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         }
     }
     // suspended mass of the ext cells, up to KC classes per batch of loads
-    constexpr int KC = 8;
+    constexpr int KC = sizeof(R) == 4 ? 4 : 8;  // FP32 keeps fewer in flight to stay at 5 waves per SIMD
     R ch_r[KC][EIT];
 #pragma unroll
     for (int kk = 0; kk < KC; kk++)
@@ -417,7 +461,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             }
             R u = R(0), v = R(0);
             if (h > hfilm) {
-                const R ih = R(1) / h;
+                const R ih = mrcp(h);
                 ih_r[r] = ih;
                 u = hu * ih;
                 v = hv * ih;
@@ -512,15 +556,34 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             xface_g(il + 1, jl, gz, cl);
             yface_g(il, jl + 1, gz, cl);
         }
-        // faces on the tile's low edges belong to the cell outside (or to the ghost,
-        // which takes the GZ of the first cell, SWEHR.c:1054-1088)
-        if (il == 0 && gj < ny && i0 <= nx) {
-            const R gz = gz_lag(i0 > 0 ? i0 - 1 : 0, gj);
-            xface_g(0, jl, gz, msqrt(gz * s_h[jl + 1]));
-        }
-        if (jl == 0 && gi < nx && j0 <= ny) {
-            const R gz = gz_lag(gi, j0 > 0 ? j0 - 1 : 0);
-            yface_g(il, 0, gz, msqrt(gz * s_h[(il + 1) * EJ]));
+    }
+    {
+        // faces on the tile's low edges belong to the cell outside (or to the ghost, which
+        // takes the GZ of the first cell, SWEHR.c:1054-1088): TJ x faces then TI y faces,
+        // one per thread from thread 0, in one pass of the first wave(s)
+        const int ef = tid;
+        if (ef < TJ + TI) {
+            const bool isx = ef < TJ;
+            const int fj = isx ? ef : 0, fi = isx ? 0 : ef - TJ;
+            const bool ok = isx ? (j0 + fj < ny) : (i0 + fi < nx);
+            if (ok) {
+                const int el = isx ? fj + 1 : (fi + 1) * EJ, er = isx ? el + EJ : el + 1;
+                const R *un = isx ? s_u : s_v, *ut = isx ? s_v : s_u;
+                const R gz = isx ? gz_lag(i0 > 0 ? i0 - 1 : 0, j0 + fj) : gz_lag(i0 + fi, j0 > 0 ? j0 - 1 : 0);
+                R f0, f1, f2;
+                face_flux_v(s_h[el], s_h[er], un[el], un[er], ut[el], ut[er], msqrt(gz * s_h[el]), msqrt(gz * s_h[er]),
+                            gz, f0, f1, f2);
+                if (isx) {
+                    s_fx[fj] = f0;
+                    s_fx[NFX + fj] = f1;
+                    s_fx[2 * NFX + fj] = f2;
+                } else {
+                    const int f = fi * (TJ + 1);
+                    s_fy[f] = f0;
+                    s_fy[NFY + f] = f2;      // hu flux (tangential)
+                    s_fy[2 * NFY + f] = f1;  // hv flux (normal), SWEHR.c:1216-1218
+                }
+            }
         }
     }
     __syncthreads();
@@ -555,7 +618,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         R gmx = R(0), gmy = R(0);
         const bool gx_in = gi > 0 && gi < nx - 1, gy_in = gj > 0 && gj < ny - 1;
         if (gx_in || gy_in) {
-            const R fac = (rhos - rho) * gz * h * h / (R(2) * rho) * R(.5) * odx;
+            const R fac = (rhos - rho) * gz * h * h * mrcp(R(2) * rho) * R(.5) * odx;
             if (gx_in) gmx -= fac * (s_ts[e + EJ] - s_ts[e - EJ]);
             if (gy_in) gmy -= fac * (s_ts[e + 1] - s_ts[e - 1]);
         }
@@ -601,9 +664,9 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         const R totm_old = totm_o;
         const R hvert = h * (n2 * rn);
         const bool deep_v = hvert > h0;
-        const R ihv = deep_v ? R(1) / hvert : R(0);
+        const R ihv = deep_v ? mrcp(hvert) : R(0);
         const R mtstar = deep_v ? R(A.mtstar0) * (h0 * ihv) : R(A.mtstar0);
-        const R H = mfmin(R(1), totm_old / mtstar);
+        const R H = mfmin(R(1), totm_old * mrcp(mtstar));  // mtstar = 0: NaN, which fmin drops as inf/NaN
         const R shield = deep_v ? h0 * ihv : R(1);
         const R rough = R(A.roughness);
         const bool active = h > R(10) * hfilm && ts < R(0.6);
@@ -625,10 +688,10 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             f1 = oneH * R(A.a) * Rr * shield;
             e2ok = totm_old > R(1e-8);
             e4ok = totm_old > R(1e-6);
-            const R itm = e2ok ? R(1) / totm_old : R(0);
+            const R itm = e2ok ? mrcp(totm_old) : R(0);
             f2 = H * itm * R(A.ad) * Rr * shield;
             f3 = oneH * R(A.F) / R(A.J) * omega;
-            if (e4ok) f4 = (R(1) - R(kBetaBedload)) * H * itm * R(A.F) / ((rhos - rho) * inv_rhos * gz * h) * omega;
+            if (e4ok) f4 = (R(1) - R(kBetaBedload)) * H * itm * R(A.F) * mrcp((rhos - rho) * inv_rhos * gz * h) * omega;
         }
 
         // conservative update (SWEHR.c:1424-1441)
@@ -645,11 +708,12 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         {
             const double ks = ks_o;
             const double vin = vinf_o;
-            const double zf = vin / (A.thetas - A.theta0);
+            const double zf = vin * mrcp(A.thetas - A.theta0);
             const double hd = (double)hn;
             // the film depth as stored in this precision, so a dry cell stays exactly dry
-            const double infl = mfmin(hd - (double)hfilm, dtd * ks * (zf + A.hf + hd) / zf);
-            A.vinf[c] = vin + infl;
+            // zf = 0 gives NaN where division gives inf (or NaN): fmin returns h - hfilm for both
+            const double infl = mfmin(hd - (double)hfilm, dtd * ks * (zf + A.hf + hd) * mrcp(zf));
+            vinfp[c] = vin + infl;
             hn = R(hd - infl);
         }
 
@@ -700,7 +764,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             } else {
                 chtemp = u;
                 const R red = mfmin(m, dt * (e2 + e4));
-                const R decay = mexp(-vf * hdt / hn);
+                const R decay = mexp(-vf * hdt * mrcp(hn));
                 u = decay * u;
                 u = u + dt * (e1 + e3) + red;
                 u = decay * u;
@@ -720,8 +784,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             bud[B_EXCH] += (double)dh;
             totm += m;
             depn += m * inv_rhos;
-            A.ch[q][k * plane + c] = u;
-            A.M[k * plane + c] = m;
+            chout[k * plane + c] = u;
+            Mp[k * plane + c] = m;
         };
         // Two classes per trip with two register sets, each refilled for class k + 2
         // as soon as it is consumed: the loads of a class are in flight for two class
@@ -732,7 +796,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             const R ch_k = ch_a, m_k = m_a;
             if (k + 1 < S) {
                 ch_a = chin[(k + 1) * plane + c];
-                m_a = A.M[(k + 1) * plane + c];
+                m_a = Mp[(k + 1) * plane + c];
             }
             update_class(k, ch_k, m_k);
         }
@@ -743,16 +807,16 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             hn = hfilm;
             hun = R(0);
             hvn = R(0);
-            R *chq = A.ch[q];
-            R u_a = chq[c], mm_a = A.M[c];
+            glob<R> *chq = chout;
+            R u_a = chq[c], mm_a = Mp[c];
             for (int k = 0; k < S; k++) {
                 const R u = u_a, mm = mm_a;
                 if (k + 1 < S) {
                     u_a = chq[(k + 1) * plane + c];
-                    mm_a = A.M[(k + 1) * plane + c];
+                    mm_a = Mp[(k + 1) * plane + c];
                 }
                 zn_c = zn_c + (double)(kbed * u);
-                A.M[k * plane + c] = mm + u;
+                Mp[k * plane + c] = mm + u;
                 chq[k * plane + c] = R(0);
             }
         }
@@ -760,22 +824,22 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         R v1 = R(0), v2 = R(0);
         if (hn > hfilm) {
             const R sf2 = manning2_over_hpow<2>(hn, rough) * gz * msqrt(hun * hun + hvn * hvn);
-            const R den = R(1) / (R(1) + dt * sf2);
+            const R den = mrcp(R(1) + dt * sf2);
             hun = hun * den;
             hvn = hvn * den;
-            const R ihn = R(1) / hn;
+            const R ihn = mrcp(hn);
             v1 = hun * ihn;
             v2 = hvn * ihn;
         }
         const R speed = msqrt(v1 * v1 + v2 * v2);
-        if (speed > mv_o) A.maxvel[c] = speed;
-        if (hn > md_o) A.maxdepth[c] = hn;
-        A.h[q][c] = hn;
-        A.hu[q][c] = hun;
-        A.hv[q][c] = hvn;
-        A.z[(p3 + 1) % 3][c] = zn_c;
-        A.totm[c] = totm;
-        A.dep[q][c] = depn;
+        if (speed > mv_o) maxvelp[c] = speed;
+        if (hn > md_o) maxdepthp[c] = hn;
+        hout[c] = hn;
+        huout[c] = hun;
+        hvout[c] = hvn;
+        zout[c] = zn_c;
+        totmp[c] = totm;
+        depout[c] = depn;
 
         // water and sediment leaving through the domain edges this step
         double wb = 0;
@@ -791,29 +855,24 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         bud[B_DET] *= a2;
     }
 
-    // ---- per-tile budget partials, accumulated across steps in the tile's own slot
-    const int lane = tid & 63, wid = tid >> 6;
+    // ---- budget partials: each wave stores its sums of this step as the pending
+    // partial of its own slot (tile, wave); the wave-speed kernel of the next step folds
+    // it into the slot's compensated running sum (fold_budget).  No barrier and no
+    // read-modify-write at the end of the tile.
+    {
+        double mine = 0;
 #pragma unroll
-    for (int k = 0; k < NB_TERMS; k++) {
-        const double v = wave_sum(bud[k]);
-        if (lane == 0) s_red[k * 16 + wid] = v;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        double *slot = A.bud + (size_t)tile * NB_TERMS * 2;
         for (int k = 0; k < NB_TERMS; k++) {
-            double v = 0;
-            for (int w = 0; w < NT / 64; w++) v += s_red[k * 16 + w];
-            double s = slot[2 * k], cc = slot[2 * k + 1];
-            neumaier(s, cc, v);
-            slot[2 * k] = s;
-            slot[2 * k + 1] = cc;
+            double v = bud[k];
+            if (__ballot(v != 0.0) != 0ull) v = wave_sum(v);  // most terms are zero in most tiles
+            if (lane == k) mine = v;
         }
+        if (lane < NB_TERMS) gmem(A.bud)[(((size_t)tile * (NT / 64) + wid) * NB_TERMS + lane) * 3 + 2] = mine;
     }
 }
 
 template <typename R, int TI, int TJ>
-__global__ __launch_bounds__(TI *TJ) void step_kernel(StepArgs<R> A)
+__global__ __launch_bounds__(TI *TJ, kStepWaves<R>) void step_kernel(StepArgs<R> A)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     // XCD-aware tile order: consecutive block ids land on different XCDs, so give
@@ -831,14 +890,15 @@ template <typename R>
 __device__ __forceinline__ void derived(const StepArgs<R> &A, int p, long long c, double &h, double &v1, double &v2,
                                         double &ts)
 {
-    const R hh = A.h[p][c];
+    const R hh = gmem(A.h[p])[c];
     h = (double)hh;
     v1 = v2 = ts = 0;
     if (hh > R(kHfilm)) {
-        v1 = (double)(A.hu[p][c] / hh);
-        v2 = (double)(A.hv[p][c] / hh);
+        v1 = (double)(gmem(A.hu[p])[c] / hh);
+        v2 = (double)(gmem(A.hv[p])[c] / hh);
         R s = R(0);
-        for (int k = 0; k < A.S; k++) s += A.ch[p][k * A.plane + c] / hh / R(kRhos);
+        const glob<R> *ch = gmem(A.ch[p]);
+        for (int k = 0; k < A.S; k++) s += ch[k * A.plane + c] / hh / R(kRhos);
         ts = (double)s;
     }
 }
@@ -847,7 +907,7 @@ __device__ __forceinline__ void derived(const StepArgs<R> &A, int p, long long c
 template <typename R>
 __device__ __forceinline__ void control_body(const StepArgs<R> &A)
 {
-    DevCtrl *C = A.ctrl;
+    glob<DevCtrl> *C = gmem(A.ctrl);
     if (!step_active(C)) return;
     __shared__ double s_best[256];
     __shared__ int s_idx[256];
@@ -868,7 +928,7 @@ __device__ __forceinline__ void control_body(const StepArgs<R> &A)
         double best = 0;
         int idx = 0x7fffffff;
         for (int i = threadIdx.x; i < A.nx; i += blockDim.x) {
-            const double hv = (double)A.h[q][(long long)i * A.py];
+            const double hv = (double)gmem(A.h[q])[(long long)i * A.py];
             if (hv > best) { best = hv; idx = i; }
         }
         // block arg-max: larger depth wins, the lower index on equal depths
@@ -895,7 +955,7 @@ __device__ __forceinline__ void control_body(const StepArgs<R> &A)
         // at (i = 2, j = 1, 10, ..., 50) (SWEHR.c:1605-1644)
         const int tid = threadIdx.x;
         if (r < cap) {
-            double *st = A.stage;
+            glob<double> *st = gmem(A.stage);
             const long long co = (long long)outlet * A.py;
             if (tid == 0) {
                 double h, v1, v2, ts;
@@ -906,8 +966,8 @@ __device__ __forceinline__ void control_body(const StepArgs<R> &A)
                 st[3 * cap + r] = ts;
             } else if (tid >= 1 && tid <= 10) {
                 const int k = tid - 1;
-                st[(4 + k) * cap + r] = k < A.S ? (double)A.M[k * A.plane + co] : 0.0;
-                st[(14 + k) * cap + r] = k < A.S ? (double)A.ch[q][k * A.plane + co] : 0.0;
+                st[(4 + k) * cap + r] = k < A.S ? (double)gmem(A.M)[k * A.plane + co] : 0.0;
+                st[(14 + k) * cap + r] = k < A.S ? (double)gmem(A.ch[q])[k * A.plane + co] : 0.0;
             } else if (tid >= 11 && tid <= 16) {
                 const int n = tid - 11;
                 const int yp = n == 0 ? 1 : 10 * n;
@@ -918,9 +978,10 @@ __device__ __forceinline__ void control_body(const StepArgs<R> &A)
                     d = hh;
                     sp = sqrt(w1 * w1 + w2 * w2);
                 }
-                A.saveflow[(3 * n + 0) * cap + r] = d;
-                A.saveflow[(3 * n + 1) * cap + r] = sp;
-                A.saveflow[(3 * n + 2) * cap + r] = tt;
+                glob<double> *sf = gmem(A.saveflow);
+                sf[(3 * n + 0) * cap + r] = d;
+                sf[(3 * n + 1) * cap + r] = sp;
+                sf[(3 * n + 2) * cap + r] = tt;
             }
         }
         if (tid == 0) {
@@ -981,8 +1042,9 @@ __global__ __launch_bounds__(TI *TJ) void batch_wave_kernel(const StepArgs<R> *a
 }
 
 template <typename R, int TI, int TJ>
-__global__ __launch_bounds__(TI *TJ) void batch_step_kernel(const StepArgs<R> *args, const TileRef *map, int n,
-                                                            int per_xcd)
+__global__ __launch_bounds__(TI *TJ, kStepWaves<R>) void batch_step_kernel(const StepArgs<R> *__restrict__ args,
+                                                                           const TileRef *__restrict__ map, int n,
+                                                                           int per_xcd)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int b = blockIdx.x;

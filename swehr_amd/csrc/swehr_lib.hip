@@ -214,10 +214,8 @@ __global__ __launch_bounds__(256) void budget_kernel(StepArgs<R> A, double *out)
     }
     for (long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x; b < A.bud_slots;
          b += (long long)gridDim.x * blockDim.x)
-        for (int k = 0; k < NB_TERMS; k++) {
-            neumaier(s[5 + k], c[5 + k], A.bud[b * NB_TERMS * 2 + 2 * k]);
-            neumaier(s[5 + k], c[5 + k], A.bud[b * NB_TERMS * 2 + 2 * k + 1]);
-        }
+        for (int k = 0; k < NB_TERMS; k++)
+            for (int m = 0; m < 3; m++) neumaier(s[5 + k], c[5 + k], A.bud[(b * NB_TERMS + k) * 3 + m]);
     __shared__ double red[10][256];
     for (int n = 0; n < 10; n++) red[n][threadIdx.x] = s[n] + c[n];
     __syncthreads();
@@ -290,7 +288,7 @@ size_t plan(swehr_handle *h, char *base, bool ks_raster)
     const size_t o_ks = ks_raster ? L.add(pd) : 0;
     const size_t o_scr = L.add(sizeof(double) * (size_t)h->nx * h->ny);
     const size_t o_rain = L.add(sizeof(double) * (size_t)h->p.rnum);
-    const size_t o_bud = L.add(sizeof(double) * (size_t)h->bud_slots * NB_TERMS * 2);
+    const size_t o_bud = L.add(sizeof(double) * (size_t)h->bud_slots * NB_TERMS * 3);
     const size_t o_stage = L.add(sizeof(double) * (size_t)kStageCols * h->stage_cap);
     const size_t o_flow = L.add(sizeof(double) * (size_t)kFlowCols * h->stage_cap);
     const size_t o_part = L.add(sizeof(double) * 10 * (size_t)h->budget_blocks);
@@ -337,8 +335,9 @@ void geometry(swehr_handle *h, int tile)
     h->lds = h->precision == SWEHR_FP64 ? tile_lds<double>(tile, h->S) : tile_lds<float>(tile, h->S);
     int slots = 0;
     for (int id = 1; id <= kNumTiles; id++) {
-        const int n = ((h->nx + kTiles[id - 1].ti - 1) / kTiles[id - 1].ti) * ((h->ny + kTiles[id - 1].tj - 1) / kTiles[id - 1].tj);
-        slots = std::max(slots, n);
+        const TileShape &t = kTiles[id - 1];
+        const int n = ((h->nx + t.ti - 1) / t.ti) * ((h->ny + t.tj - 1) / t.tj) * (t.ti * t.tj / 64);
+        slots = std::max(slots, n);  // one per wave
     }
     h->bud_slots = slots;
 }
