@@ -9,8 +9,9 @@ synchronisations.  ``value`` is cell-class-steps/s = cells x size classes x step
 
 Roofline: the fused step kernel's average duration is measured live (HIP events around
 eagerly launched steps); its algorithmic HBM bytes per launch are cells x B_cell with
-B_cell = 2 w (5 + 2 S) + 40 bytes (one read and one write of h, hu, hv, MAXDEPTH, MAXVEL,
-ch_k, M_k at w bytes, of the bed z and VINF as FP64, one read of the FP64 Ks raster).  ``traffic`` is
+B_cell = 2 w (6 + 2 S) + 16 + w bytes (one read and one write of h, hu, hv, z, MAXDEPTH,
+MAXVEL, ch_k, M_k at w bytes, VINF as FP64, one read of the Ks raster).  The FP32 form
+keeps z and the Ks raster in FP64, 12 B per cell more than this count credits it with.  ``traffic`` is
 the step kernel's measured HBM bytes per launch, read from profiles/traffic.json (made
 with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs scaled by a copy-kernel calibration).
 
@@ -69,9 +70,35 @@ class Hip:
     def sync(self):
         self._ok(self.L.hipDeviceSynchronize(), "hipDeviceSynchronize")
 
+    def copy_bandwidth(self, mib: int = 1024, reps: int = 5) -> float:
+        """Device-to-device copy of mib MiB, reps times, timed with events: GB/s of HBM
+        traffic (bytes read + bytes written), the measured counterpart of the 8 TB/s spec."""
+        n = mib << 20
+        a, b = C.c_void_p(), C.c_void_p()
+        self.L.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.L.hipFree.argtypes = [C.c_void_p]
+        self.L.hipMemsetD8.argtypes = [C.c_void_p, C.c_ubyte, C.c_size_t]
+        self.L.hipMemcpyDtoDAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]
+        self._ok(self.L.hipMalloc(C.byref(a), n), "hipMalloc")
+        self._ok(self.L.hipMalloc(C.byref(b), n), "hipMalloc")
+        try:
+            self._ok(self.L.hipMemsetD8(a, 1, n), "hipMemsetD8")
+            self._ok(self.L.hipMemcpyDtoDAsync(b, a, n, None), "hipMemcpyDtoDAsync")  # warm
+            e0, e1 = self.event(), self.event()
+            self.sync()
+            self.record(e0, 0)
+            for _ in range(reps):
+                self._ok(self.L.hipMemcpyDtoDAsync(b, a, n, None), "hipMemcpyDtoDAsync")
+            self.record(e1, 0)
+            ms = self.elapsed_ms(e0, e1)
+        finally:
+            self.L.hipFree(a)
+            self.L.hipFree(b)
+        return 2.0 * n * reps / (ms * 1e-3) / 1e9
+
 
 def bytes_per_cell(S: int, w: int, ks_raster: bool = True) -> int:
-    return 2 * w * (5 + 2 * S) + 32 + (8 if ks_raster else 0)
+    return 2 * w * (6 + 2 * S) + 16 + (w if ks_raster else 0)
 
 
 def timed_run(hip, solver, steps):
@@ -105,6 +132,19 @@ def cpu_baseline(case, budget_s: float, threads: int):
     el = one + (time.perf_counter() - t0)
     o.close()
     return 1 + more, el
+
+
+def host_description() -> str:
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return "%s, %d logical CPUs visible" % (model, os.cpu_count() or 0)
 
 
 def traffic_for(key: str):
@@ -368,14 +408,20 @@ def main():
                 "kernel": "step_kernel", "kernel_ms": round(step_ms, 4), "bytes_per_cell": bcell,
                 "cells_per_launch": cells}
 
+    roofline["hbm_copy_measured_GBs"] = round(hip.copy_bandwidth(), 1)
+    log("device copy: %.1f GB/s" % roofline["hbm_copy_measured_GBs"])
+
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         ncpu, el = cpu_baseline(case, args.cpu_seconds, threads)
+        n1, el1 = cpu_baseline(case, args.cpu_seconds / 3, 1)
         cpu = {"value": cells * S * ncpu / el, "unit": "cell-class-steps/s", "cores": threads, "kind": "port",
                "sample": "FP64 C/OpenMP oracle, %d steps of the %dx%d case from its initial state (t=0), %.1f s; "
                          "GPU on the same first 20 steps: %.4e cell-class-steps/s"
-                         % (ncpu, args.grid, args.grid, el, cells * S * n_first / (ms_first * 1e-3))}
+                         % (ncpu, args.grid, args.grid, el, cells * S * n_first / (ms_first * 1e-3)),
+               "one_thread": {"value": cells * S * n1 / el1, "steps": n1, "seconds": round(el1, 2)},
+               "host": host_description()}
         log("cpu baseline: %s" % cpu)
 
     forms = {}
