@@ -101,6 +101,20 @@ __device__ __forceinline__ glob<T> *gmem(T *p)
     return (glob<T> *)p;
 }
 
+// Element at byte offset `off` of a plane.  Offsets are 32-bit (a plane is < 4 GiB,
+// checked at create): the plane base stays in scalar registers and the cell's one
+// offset register serves every plane of the same element size.
+template <typename T>
+__device__ __forceinline__ glob<T> &at(glob<T> *base, unsigned off)
+{
+    return *(glob<T> *)((glob<char> *)base + off);
+}
+template <typename T>
+__device__ __forceinline__ const glob<T> &at(const glob<T> *base, unsigned off)
+{
+    return *(const glob<T> *)((const glob<char> *)base + off);
+}
+
 __device__ __forceinline__ bool step_active(const glob<DevCtrl> *C)
 {
     return !C->done && !C->print_due && C->steps < C->step_limit;
@@ -193,43 +207,32 @@ __device__ __forceinline__ R wave_speed_cell(const StepArgs<R> &A, const glob<R>
                                              const glob<R> *hv, const glob<double> *zp, int i, int j,
                                              const InflowState &in)
 {
-    const int nx = A.nx, ny = A.ny, py = A.py;
+    const int nx = A.nx, ny = A.ny;
+    const unsigned py = (unsigned)A.py, c = (unsigned)i * py + (unsigned)j;
+    // neighbours clamped into the grid, so every load is unconditional; the one-sided
+    // differences at the edges (SWEHR.c:1240-1277) then read the cell itself
+    const unsigned im = i > 0 ? c - py : c, ip = i < nx - 1 ? c + py : c;
+    const unsigned jm = j > 0 ? c - 1 : c, jp = j < ny - 1 ? c + 1 : c;
     const double odx = 1.0 / A.dx;
-    const long long c = (long long)i * py + j;
-    double sxd, syd;
-    if (i != 0 && i != nx - 1) sxd = 0.5 * odx * (zp[c + py] - zp[c - py]);
-    else if (i == nx - 1) sxd = odx * (zp[c] - zp[i > 0 ? c - py : c]);
-    else sxd = odx * (zp[nx > 1 ? c + py : c] - zp[c]);
-    if (j != 0 && j != ny - 1) syd = 0.5 * odx * (zp[c + 1] - zp[c - 1]);
-    else if (j == ny - 1) syd = odx * (zp[c] - zp[j > 0 ? c - 1 : c]);
-    else syd = odx * (zp[ny > 1 ? c + 1 : c] - zp[c]);
+    const double fx = (i != 0 && i != nx - 1) ? 0.5 * odx : odx;
+    const double fy = (j != 0 && j != ny - 1) ? 0.5 * odx : odx;
+    const double sxd = fx * (at(zp, ip * 8u) - at(zp, im * 8u));
+    const double syd = fy * (at(zp, jp * 8u) - at(zp, jm * 8u));
+    const unsigned w = (unsigned)sizeof(R);
+    const R hc = at(h, c * w), huc = at(hu, c * w), hvc = at(hv, c * w);
+    R hx = at(h, ip * w), qx = at(hu, ip * w);
+    R hy = at(h, jp * w), qy = at(hv, jp * w);
     const R sx = R(sxd), sy = R(syd);
     const R g = R(kG) * mrsqrt(sx * sx + sy * sy + R(1));
-    const R hc = h[c], huc = hu[c], hvc = hv[c];
-    R smax = R(0), s;
-    {
-        R hr, qr;
-        if (i < nx - 1) { hr = h[c + py]; qr = hu[c + py]; }
-        else { hr = hc; qr = A.bndry[2] == 1 ? -huc : huc; }
-        s = face_speed(hc, hr, huc, qr, g);
-        smax = s > smax ? s : smax;
+    if (i == nx - 1) qx = A.bndry[2] == 1 ? -huc : huc;  // hx = hc already
+    if (j == ny - 1) {
+        if (A.inflow) { hy = R(in.h); qy = R(in.hv); }
+        else qy = A.bndry[3] == 1 ? -hvc : hvc;
     }
-    {
-        R hr, qr;
-        if (j < ny - 1) { hr = h[c + 1]; qr = hv[c + 1]; }
-        else if (A.inflow) { hr = R(in.h); qr = R(in.hv); }
-        else { hr = hc; qr = A.bndry[3] == 1 ? -hvc : hvc; }
-        s = face_speed(hc, hr, hvc, qr, g);
-        smax = s > smax ? s : smax;
-    }
-    if (i == 0) {  // face 1/2: ghost on the left, GZ of cell 1 (SWEHR.c:1054-1059)
-        s = face_speed(hc, hc, A.bndry[0] == 1 ? -huc : huc, huc, g);
-        smax = s > smax ? s : smax;
-    }
-    if (j == 0) {
-        s = face_speed(hc, hc, A.bndry[1] == 1 ? -hvc : hvc, hvc, g);
-        smax = s > smax ? s : smax;
-    }
+    R smax = mfmax(face_speed(hc, hx, huc, qx, g), face_speed(hc, hy, hvc, qy, g));
+    if (i == 0)  // face 1/2: ghost on the left, GZ of cell 1 (SWEHR.c:1054-1059)
+        smax = mfmax(smax, face_speed(hc, hc, A.bndry[0] == 1 ? -huc : huc, huc, g));
+    if (j == 0) smax = mfmax(smax, face_speed(hc, hc, A.bndry[1] == 1 ? -hvc : hvc, hvc, g));
     return smax;
 }
 
@@ -248,9 +251,10 @@ __device__ __forceinline__ void block_max_to(unsigned long long *slots, int slot
     }
 }
 
-// One thread per cell on 4 x 64 tiles (a wave reads one 64-cell row segment of each
-// field); tiles are dealt to XCDs in contiguous runs so that the i+1 row a tile reads
-// is in the same XCD's L2.  The grid is the tile count rounded up to a multiple of 8.
+// Two cells per thread on 8 x 64 tiles (rows il and il + 4; a wave reads 64-cell row
+// segments of each field); tiles are dealt to XCDs in contiguous runs so that the
+// next row a tile reads is in the same XCD's L2.  The grid is the tile count rounded
+// up to a multiple of 8.
 template <typename R>
 __global__ __launch_bounds__(256) void wave_speed_kernel(StepArgs<R> A)
 {
@@ -259,15 +263,19 @@ __global__ __launch_bounds__(256) void wave_speed_kernel(StepArgs<R> A)
     const long long steps = C->steps;
     const int p = (int)(steps & 1), p3 = (int)(steps % 3);
     const InflowState in = ctrl_inflow(C);
-    const int tiles_j = (A.ny + 63) >> 6, ntiles = ((A.nx + 3) >> 2) * tiles_j;
+    const int tiles_j = (A.ny + 63) >> 6, ntiles = ((A.nx + 7) >> 3) * tiles_j;
     const int b = blockIdx.x, per = (int)(gridDim.x >> 3);
     const int tile = (b & 7) * per + (b >> 3);
     R s = R(0);
     if (tile < ntiles) {
         const int ti = tile / tiles_j, tj = tile - ti * tiles_j;
-        const int i = ti * 4 + (int)(threadIdx.x >> 6), j = tj * 64 + (int)(threadIdx.x & 63);
-        if (i < A.nx && j < A.ny)
-            s = wave_speed_cell(A, gmem(A.h[p]), gmem(A.hu[p]), gmem(A.hv[p]), gmem(A.z[(p3 + 2) % 3]), i, j, in);
+        const int i = ti * 8 + (int)(threadIdx.x >> 6), j = tj * 64 + (int)(threadIdx.x & 63);
+        const glob<R> *h = gmem(A.h[p]), *hu = gmem(A.hu[p]), *hv = gmem(A.hv[p]);
+        const glob<double> *zp = gmem(A.z[(p3 + 2) % 3]);
+        if (j < A.ny) {
+            if (i < A.nx) s = wave_speed_cell(A, h, hu, hv, zp, i, j, in);
+            if (i + 4 < A.nx) s = mfmax(s, wave_speed_cell(A, h, hu, hv, zp, i + 4, j, in));
+        }
     }
     // the step kernel's budget partials of the previous step
     for (long long g = (long long)b * blockDim.x + threadIdx.x; g < (long long)A.bud_slots * NB_TERMS;
@@ -369,22 +377,26 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     const int il = tid / TJ, jl = tid - (tid / TJ) * TJ;
     const int gi = i0 + il, gj = j0 + jl;
     const bool own = gi < nx && gj < ny;
-    const long long c = (long long)gi * py + gj;
+    // cell offsets within a plane are 32-bit unsigned (a plane holds < 2^32 values on any
+    // GPU memory): with the plane base in scalar registers every access is base + one
+    // shared offset register, with no per-access 64-bit address arithmetic
+    const unsigned c = (unsigned)gi * (unsigned)py + (unsigned)gj;
+    const unsigned cR = c * (unsigned)sizeof(R), c8 = c * 8u;  // byte offsets
     R hu_o = R(0), hv_o = R(0), totm_o = R(0), mv_o = R(0), md_o = R(0);
     double vinf_o = 0, ks_o = A.ks;
     const int lane = tid & 63, wid = tid >> 6;
     // ch_k and M_k of the first class (the class loop prefetches one class ahead)
     R ch_a = R(0), m_a = R(0);
     if (own) {
-        hu_o = huin[c];
-        hv_o = hvin[c];
-        totm_o = totmp[c];
-        mv_o = maxvelp[c];
-        md_o = maxdepthp[c];
-        vinf_o = vinfp[c];
-        if (A.ks_raster) ks_o = gmem(A.ks_raster)[c];
-        ch_a = chin[c];
-        m_a = Mp[c];
+        hu_o = at(huin, cR);
+        hv_o = at(hvin, cR);
+        totm_o = at(totmp, cR);
+        mv_o = at(maxvelp, cR);
+        md_o = at(maxdepthp, cR);
+        vinf_o = at(vinfp, c8);
+        if (A.ks_raster) ks_o = at(gmem(A.ks_raster), c8);
+        ch_a = at(chin, cR);
+        m_a = at(Mp, cR);
 
     }
 
@@ -399,13 +411,13 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             int ggi = i0 - 2 + zi, ggj = j0 - 2 + zj;
             ggi = ggi < 0 ? 0 : (ggi >= nx ? nx - 1 : ggi);
             ggj = ggj < 0 ? 0 : (ggj >= ny ? ny - 1 : ggj);
-            const long long cz = (long long)ggi * py + ggj;
-            zc_r[r] = zc[cz];
-            zp_r[r] = zp[cz];
+            const unsigned cz = (unsigned)ggi * (unsigned)py + (unsigned)ggj;
+            zc_r[r] = at(zc, cz * 8u);
+            zp_r[r] = at(zp, cz * 8u);
         }
     }
     // ext cell r of this thread: clamped source cell, and its state
-    long long c_r[EIT];
+    unsigned c_r[EIT];
     R h_r[EIT], hu_r[EIT], hv_r[EIT], dep_r[EIT];
 #pragma unroll
     for (int r = 0; r < EIT; r++) {
@@ -416,11 +428,11 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             const int ggi = i0 - 1 + ei, ggj = j0 - 1 + ej;
             const int ci = ggi < 0 ? 0 : (ggi >= nx ? nx - 1 : ggi);
             const int cj = ggj < 0 ? 0 : (ggj >= ny ? ny - 1 : ggj);
-            c_r[r] = (long long)ci * py + cj;
-            h_r[r] = hin[c_r[r]];
-            hu_r[r] = huin[c_r[r]];
-            hv_r[r] = hvin[c_r[r]];
-            dep_r[r] = depin[c_r[r]];
+            c_r[r] = ((unsigned)ci * (unsigned)py + (unsigned)cj) * (unsigned)sizeof(R);  // bytes
+            h_r[r] = at(hin, c_r[r]);
+            hu_r[r] = at(huin, c_r[r]);
+            hv_r[r] = at(hvin, c_r[r]);
+            dep_r[r] = at(depin, c_r[r]);
         }
     }
     // suspended mass of the ext cells, up to KC classes per batch of loads
@@ -430,7 +442,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     for (int kk = 0; kk < KC; kk++)
 #pragma unroll
         for (int r = 0; r < EIT; r++)
-            if (kk < S && tid + r * NT < EXT) ch_r[kk][r] = chin[kk * plane + c_r[r]];
+            if (kk < S && tid + r * NT < EXT) ch_r[kk][r] = at(chin + kk * plane, c_r[r]);
 
     // boundary rules and velocities of the ext cells
     const InflowState in = ctrl_inflow(C);
@@ -480,7 +492,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             for (int kk = 0; kk < KC; kk++)
 #pragma unroll
                 for (int r = 0; r < EIT; r++)
-                    if (k0 + kk < S && tid + r * NT < EXT) ch_r[kk][r] = chin[(k0 + kk) * plane + c_r[r]];
+                    if (k0 + kk < S && tid + r * NT < EXT) ch_r[kk][r] = at(chin + (k0 + kk) * plane, c_r[r]);
         }
 #pragma unroll
         for (int kk = 0; kk < KC; kk++) {
@@ -713,13 +725,15 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             // the film depth as stored in this precision, so a dry cell stays exactly dry
             // zf = 0 gives NaN where division gives inf (or NaN): fmin returns h - hfilm for both
             const double infl = mfmin(hd - (double)hfilm, dtd * ks * (zf + A.hf + hd) * mrcp(zf));
-            vinfp[c] = vin + infl;
+            at(vinfp, c8) = vin + infl;
             hn = R(hd - infl);
         }
 
         // sediment: flux divergence, sources and exchange per class (SWEHR.c:1454-1511)
         const R fx0l = s_fx[fxl], fx0r = s_fx[fxr], fy0l = s_fy[fyl], fy0r = s_fy[fyr];
-        const int exl = e - EJ, exr = e + EJ, eyl = e - 1, eyr = e + 1;
+        // upwind cell of each face (SWEHR.c:1152-1157, 1219-1224): class-invariant
+        const int uxl = fx0l > R(0) ? e - EJ : e, uxr = fx0r > R(0) ? e : e + EJ;
+        const int uyl = fy0l > R(0) ? e - 1 : e, uyr = fy0r > R(0) ? e : e + 1;
         const R rsp = rhos * (R(1) - R(kPhi));
         const R kbed = R(1) / rsp;
         const R hdt = R(0.5) * dt;
@@ -729,11 +743,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         // class k from its suspended mass ch_k and deposited mass m_k
         auto update_class = [&](int k, R ch_k, R m_k) {
             const R *sk = s_sed + k * EXT;
-            const R sc = sk[e];
-            const R fkxl = fx0l > R(0) ? fx0l * sk[exl] : fx0l * sc;
-            const R fkxr = fx0r > R(0) ? fx0r * sc : fx0r * sk[exr];
-            const R fkyl = fy0l > R(0) ? fy0l * sk[eyl] : fy0l * sc;
-            const R fkyr = fy0r > R(0) ? fy0r * sc : fy0r * sk[eyr];
+            const R fkxl = fx0l * sk[uxl], fkxr = fx0r * sk[uxr];
+            const R fkyl = fy0l * sk[uyl], fkyr = fy0r * sk[uyr];
             if (gi == 0) sb_out -= (double)fkxl;
             if (gi == nx - 1) sb_out += (double)fkxr;
             if (gj == 0) sb_out -= (double)fkyl;
@@ -784,8 +795,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             bud[B_EXCH] += (double)dh;
             totm += m;
             depn += m * inv_rhos;
-            chout[k * plane + c] = u;
-            Mp[k * plane + c] = m;
+            at(chout + k * plane, cR) = u;
+            at(Mp + k * plane, cR) = m;
         };
         // Two classes per trip with two register sets, each refilled for class k + 2
         // as soon as it is consumed: the loads of a class are in flight for two class
@@ -795,8 +806,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         for (int k = 0; k < S; k++) {
             const R ch_k = ch_a, m_k = m_a;
             if (k + 1 < S) {
-                ch_a = chin[(k + 1) * plane + c];
-                m_a = Mp[(k + 1) * plane + c];
+                ch_a = at(chin + (k + 1) * plane, cR);
+                m_a = at(Mp + (k + 1) * plane, cR);
             }
             update_class(k, ch_k, m_k);
         }
@@ -808,16 +819,16 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             hun = R(0);
             hvn = R(0);
             glob<R> *chq = chout;
-            R u_a = chq[c], mm_a = Mp[c];
+            R u_a = at(chq, cR), mm_a = at(Mp, cR);
             for (int k = 0; k < S; k++) {
                 const R u = u_a, mm = mm_a;
                 if (k + 1 < S) {
-                    u_a = chq[(k + 1) * plane + c];
-                    mm_a = Mp[(k + 1) * plane + c];
+                    u_a = at(chq + (k + 1) * plane, cR);
+                    mm_a = at(Mp + (k + 1) * plane, cR);
                 }
                 zn_c = zn_c + (double)(kbed * u);
-                Mp[k * plane + c] = mm + u;
-                chq[k * plane + c] = R(0);
+                at(Mp + k * plane, cR) = mm + u;
+                at(chq + k * plane, cR) = R(0);
             }
         }
         // implicit friction (SWEHR.c:1553-1565) and maxima (SWEHR.c:1587-1588)
@@ -832,14 +843,14 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             v2 = hvn * ihn;
         }
         const R speed = msqrt(v1 * v1 + v2 * v2);
-        if (speed > mv_o) maxvelp[c] = speed;
-        if (hn > md_o) maxdepthp[c] = hn;
-        hout[c] = hn;
-        huout[c] = hun;
-        hvout[c] = hvn;
-        zout[c] = zn_c;
-        totmp[c] = totm;
-        depout[c] = depn;
+        if (speed > mv_o) at(maxvelp, cR) = speed;
+        if (hn > md_o) at(maxdepthp, cR) = hn;
+        at(hout, cR) = hn;
+        at(huout, cR) = hun;
+        at(hvout, cR) = hvn;
+        at(zout, c8) = zn_c;
+        at(totmp, cR) = totm;
+        at(depout, cR) = depn;
 
         // water and sediment leaving through the domain edges this step
         double wb = 0;

@@ -330,7 +330,7 @@ void geometry(swehr_handle *h, int tile)
     h->nblocks = h->tiles_i * h->tiles_j;
     h->blocks_per_xcd = (h->nblocks + 7) / 8;
     const long long n = (long long)h->nx * h->ny;
-    h->k1_blocks = (int)(8 * ((((h->nx + 3) / 4) * ((h->ny + 63) / 64) + 7) / 8));  // 4 x 64 tiles
+    h->k1_blocks = (int)(8 * ((((h->nx + 7) / 8) * ((h->ny + 63) / 64) + 7) / 8));  // 8 x 64 tiles
     h->budget_blocks = (int)std::min<long long>((n + 255) / 256, 1024);
     h->lds = h->precision == SWEHR_FP64 ? tile_lds<double>(tile, h->S) : tile_lds<float>(tile, h->S);
     int slots = 0;
@@ -443,6 +443,11 @@ int check_params(const swehr_params *p)
     if (!p || p->nx < 1 || p->ny < 1 || p->nclass < 1 || p->nclass > kMaxClasses || p->rnum < 1 || !(p->dx > 0) ||
         !(p->rint > 0) || !std::isfinite(p->tend) || !(p->cstable > 0) || !(p->thetas != p->theta0)) {
         swehr_set_error("invalid parameters%s%s", "", "");
+        return SWEHR_ERR_BAD_INPUT;
+    }
+    // the kernels address a plane with 32-bit byte offsets: at most 2^29 FP64 values
+    if ((double)p->nx * (double)pick_pitch(p->ny) * 8.0 >= 4294967296.0) {
+        swehr_set_error("grid %s too large for one instance (at most 2^29 cells per plane)%s", "", "");
         return SWEHR_ERR_BAD_INPUT;
     }
     return SWEHR_OK;

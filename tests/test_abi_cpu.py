@@ -57,3 +57,18 @@ def test_create_without_gpu_reports_no_device():
     with pytest.raises(sv.SwehrError) as ei:
         sv.Solver(case)
     assert ei.value.code == _abi.ERR_NO_DEVICE
+
+
+def test_create_rejects_a_grid_beyond_32bit_plane_offsets():
+    # 2^29 FP64 cells per plane is the limit of the kernels' 32-bit byte offsets; the
+    # check precedes any device call, so it runs without a GPU
+    L = sv.lib()
+    p = sv.default_params()
+    p.nx, p.ny, p.nclass, p.rnum, p.rint, p.tend, p.dx = 24000, 24000, 3, 1, 1.0, 1.0, 1.0
+    one = (C.c_double * 3)(1e-4, 1e-4, 1e-4)
+    rain = (C.c_double * 1)(0.0)
+    topo = (C.c_double * 1)(0.0)  # never read: the parameters are refused first
+    h = C.c_void_p()
+    rc = L.swehr_create(C.byref(p), topo, None, one, one, rain, _abi.FP64, C.byref(h))
+    assert rc == _abi.ERR_BAD_INPUT
+    assert b"too large" in L.swehr_last_error()
