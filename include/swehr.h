@@ -161,6 +161,14 @@ int swehr_download(swehr_handle *h, int field, int k, double *host);
  * saveflow[c*capacity + r] for c < SWEHR_FLOW_COLS, r < *count (SWEHR.c:1608-1675). */
 int swehr_stage(swehr_handle *h, double *stage, double *saveflow, int capacity, int *count);
 
+/* Outlet discharge at each stage record (the times of swehr_stage): the water (m^3/s)
+ * and sediment (kg/s) leaving through the j = 1/2 face - the open side with the
+ * reference's boundary types - in the step that ended at that time.  No counterpart in
+ * the reference, which records the outlet cell's depth and velocity only (SWEHR.c:
+ * 1593-1644); this is its outlet hydrograph and sedigraph.  q, qs: capacity values each
+ * (either may be NULL); count: records so far. */
+int swehr_discharge(swehr_handle *h, double *q, double *qs, int capacity, int *count);
+
 /* Global budgets, FP64 compensated sums:
  *  [0] water on the surface  sum h*dx^2            [1] water infiltrated sum (VINF-vinf0)*dx^2
  *  [2] rain added            sum dt*R*dx^2 (cells)  [3] net boundary water outflow (m^3)

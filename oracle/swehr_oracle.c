@@ -69,6 +69,8 @@ struct oracle {
     int stagecnt, status, done;
     long fail_events; /* cells flagged unstable, summed over steps (diagnostic) */
     double *stage, *saveflow;
+    double *discharge;     /* [2][SWEHR_STAGE_MAX]: outlet water (m^3/s) and sediment (kg/s) */
+    double qout, sout;     /* outflow through the j = 1/2 face in the last flux pass */
     double z0sum;
     /* budgets: Neumaier sum and compensation per term */
     double bud[SWEHR_BUDGET_N], bud_c[SWEHR_BUDGET_N];
@@ -567,6 +569,17 @@ static void boundary_budget(struct oracle *o)
     }
     nsum(&o->bud[3], &o->bud_c[3], (wq + wc) * w);
     nsum(&o->bud[7], &o->bud_c[7], (sq + scc) * w);
+    /* outlet discharge: outflow through the j = 1/2 face, the one open side by default
+     * (SWEHR.c:618-621); per unit width times dx, summed over i (not in the reference) */
+    double q = 0, qs = 0;
+    for (int i = 1; i <= nx; i++) {
+        q += -o->fy[IX(i, 1)] * o->p.dx;
+        double si = 0;
+        for (int k = 0; k < S; k++) si -= o->fy[(3 + k) * P + IX(i, 1)];
+        qs += si * o->p.dx;
+    }
+    o->qout = q;
+    o->sout = qs;
 }
 
 /* outlet and probe records, SWEHR.c:1593-1676 */
@@ -587,6 +600,8 @@ static void sample_stage(struct oracle *o)
     st[1 * SWEHR_STAGE_MAX + r] = o->h[c];
     st[2 * SWEHR_STAGE_MAX + r] = pow(o->vel1[c] * o->vel1[c] + o->vel2[c] * o->vel2[c], 0.5);
     st[3 * SWEHR_STAGE_MAX + r] = o->totsed[c];
+    o->discharge[0 * SWEHR_STAGE_MAX + r] = o->qout;
+    o->discharge[1 * SWEHR_STAGE_MAX + r] = o->sout;
     for (int k = 0; k < 10; k++) {
         st[(4 + k) * SWEHR_STAGE_MAX + r] = k < S ? o->M[k * P + c] : 0;
         st[(14 + k) * SWEHR_STAGE_MAX + r] = k < S ? o->ch[k * P + c] : 0;
@@ -694,6 +709,7 @@ oracle_t *oracle_create(const swehr_params *p, const double *topo, const double 
     o->ds = alloc0(S); o->pf = alloc0(S); o->rain = alloc0(p->rnum);
     o->stage = alloc0((long)SWEHR_STAGE_COLS * SWEHR_STAGE_MAX);
     o->saveflow = alloc0((long)SWEHR_FLOW_COLS * SWEHR_STAGE_MAX);
+    o->discharge = alloc0(2L * SWEHR_STAGE_MAX);
     memcpy(o->ds, ds, sizeof(double) * S);
     memcpy(o->pf, pfrac, sizeof(double) * S);
     memcpy(o->rain, rain, sizeof(double) * p->rnum);
@@ -760,6 +776,15 @@ int oracle_download(const oracle_t *o, int field, int k, double *host)
     return SWEHR_OK;
 }
 
+int oracle_discharge(const oracle_t *o, double *q, double *qs, int capacity, int *count)
+{
+    const int n = o->stagecnt < capacity ? o->stagecnt : capacity;
+    if (q) memcpy(q, o->discharge, sizeof(double) * n);
+    if (qs) memcpy(qs, o->discharge + SWEHR_STAGE_MAX, sizeof(double) * n);
+    if (count) *count = o->stagecnt;
+    return SWEHR_OK;
+}
+
 int oracle_stage(const oracle_t *o, double *stage, double *saveflow, int capacity, int *count)
 {
     const int n = o->stagecnt < capacity ? o->stagecnt : capacity;
@@ -806,7 +831,7 @@ void oracle_destroy(oracle_t *o)
     double *ptrs[] = {o->h, o->hu, o->hv, o->ch, o->M, o->sed, o->z, o->zprev, o->vinf, o->totm, o->dep,
                       o->maxvel, o->maxdepth, o->totsed, o->rho, o->vel1, o->vel2, o->ks, o->fx, o->fy,
                       o->sx, o->sy, o->gmx, o->gmy, o->dfm, o->btx, o->bty, o->gzn, o->E, o->vf,
-                      o->ds, o->pf, o->rain, o->stage, o->saveflow};
+                      o->ds, o->pf, o->rain, o->stage, o->saveflow, o->discharge};
     for (size_t n = 0; n < sizeof(ptrs) / sizeof(ptrs[0]); n++) free(ptrs[n]);
     free(o->fail);
     free(o);

@@ -34,6 +34,9 @@ int oracle_run(oracle_t *o, long n);
 void oracle_info(const oracle_t *o, swehr_step_info *info);
 int oracle_download(const oracle_t *o, int field, int k, double *host);
 int oracle_stage(const oracle_t *o, double *stage, double *saveflow, int capacity, int *count);
+/* outlet discharge at each stage record: water (m^3/s) and sediment (kg/s) through the
+ * j = 1/2 face in the step just taken */
+int oracle_discharge(const oracle_t *o, double *q, double *qs, int capacity, int *count);
 int oracle_budget(const oracle_t *o, double *out);
 long oracle_fail_events(const oracle_t *o);
 void oracle_set_threads(int n);

@@ -6,8 +6,11 @@
 
 #include <ctype.h>
 #include <math.h>
+#include <pthread.h>
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
 
 int swehr_params_init(swehr_params *p)
 {
@@ -29,6 +32,229 @@ int swehr_params_init(swehr_params *p)
     return SWEHR_OK;
 }
 
+/* ------------------------------------------------------------------ threads */
+
+/* Worker count for the text I/O: SWEHR_IO_THREADS, else the online CPUs, at most 16. */
+static int io_threads(void)
+{
+    const char *e = getenv("SWEHR_IO_THREADS");
+    long n = e ? strtol(e, NULL, 10) : sysconf(_SC_NPROCESSORS_ONLN);
+    if (n < 1) n = 1;
+    if (n > 16) n = 16;
+    return (int)n;
+}
+
+typedef struct par_job {
+    void (*fn)(long lo, long hi, void *ctx);
+    void *ctx;
+    long n, per;
+    int k;
+} par_job;
+
+static void *par_run(void *arg)
+{
+    par_job *j = (par_job *)arg;
+    long lo = (long)j->k * j->per, hi = lo + j->per;
+    if (hi > j->n) hi = j->n;
+    if (lo < hi) j->fn(lo, hi, j->ctx);
+    return NULL;
+}
+
+/* fn(lo, hi, ctx) over [0, n) split into contiguous ranges, one per thread */
+static void par_for(long n, void (*fn)(long, long, void *), void *ctx)
+{
+    int t = io_threads();
+    if (t > n) t = (int)(n > 0 ? n : 1);
+    if (t <= 1) {
+        if (n > 0) fn(0, n, ctx);
+        return;
+    }
+    pthread_t th[16];
+    par_job jobs[16];
+    const long per = (n + t - 1) / t;
+    int started = 0;
+    for (int k = 0; k < t; k++) {
+        jobs[k] = (par_job){fn, ctx, n, per, k};
+        if (k > 0 && pthread_create(&th[k], NULL, par_run, &jobs[k]) == 0) started |= 1 << k;
+    }
+    par_run(&jobs[0]);
+    for (int k = 1; k < t; k++) {
+        if (started & (1 << k)) pthread_join(th[k], NULL);
+        else par_run(&jobs[k]); /* thread not available: run it here */
+    }
+}
+
+/* ------------------------------------------------------------------ numbers as text */
+
+static const uint64_t kPow10[19] = {1ull, 10ull, 100ull, 1000ull, 10000ull, 100000ull, 1000000ull, 10000000ull,
+                                    100000000ull, 1000000000ull, 10000000000ull, 100000000000ull,
+                                    1000000000000ull, 10000000000000ull, 100000000000000ull,
+                                    1000000000000000ull, 10000000000000000ull, 100000000000000000ull,
+                                    1000000000000000000ull};
+
+/* Writes v as printf("%.<prec>f") does (prec 1..18) and returns the length.  The
+ * digits come from the exact binary value: v = m 2^e, N = round(|v| 10^prec) computed
+ * in 128-bit integers with ties to even, the rounding glibc applies; values whose N
+ * does not fit 64 bits, and NaN / inf, go through snprintf. */
+int swehr_io_format_fixed(char *out, double v, int prec)
+{
+    uint64_t bits;
+    memcpy(&bits, &v, sizeof bits);
+    const int neg = (int)(bits >> 63), ex = (int)((bits >> 52) & 0x7ff);
+    const uint64_t frac = bits & ((1ull << 52) - 1);
+    if (ex == 0x7ff || prec < 1 || prec > 18) return snprintf(out, SWEHR_FIXED_MAX, "%.*f", prec, v);
+    const uint64_t m = ex ? frac | (1ull << 52) : frac;
+    const int e = ex ? ex - 1075 : -1074; /* v = m 2^e */
+    const uint64_t p10 = kPow10[prec];
+    unsigned __int128 N;
+    if (m == 0) {
+        N = 0;
+    } else if (e >= 0) {
+        if (e > 60) return snprintf(out, SWEHR_FIXED_MAX, "%.*f", prec, v);
+        const unsigned __int128 a = (unsigned __int128)m * p10;
+        if (a >> (127 - e)) return snprintf(out, SWEHR_FIXED_MAX, "%.*f", prec, v);
+        N = a << e;
+    } else {
+        const int sh = -e;
+        const unsigned __int128 num = (unsigned __int128)m * p10; /* < 2^113 */
+        if (sh >= 127) {
+            N = 0; /* num < 2^113 < half */
+        } else {
+            unsigned __int128 q = num >> sh;
+            const unsigned __int128 r = num - (q << sh), half = (unsigned __int128)1 << (sh - 1);
+            if (r > half || (r == half && (q & 1))) q++;
+            N = q;
+        }
+    }
+    if (N >> 64) return snprintf(out, SWEHR_FIXED_MAX, "%.*f", prec, v);
+    const uint64_t n = (uint64_t)N, ip = n / p10, fp = n % p10;
+    char *o = out;
+    if (neg) *o++ = '-';
+    char digits[24];
+    int nd = 0;
+    uint64_t x = ip;
+    do {
+        digits[nd++] = (char)('0' + x % 10);
+        x /= 10;
+    } while (x);
+    while (nd) *o++ = digits[--nd];
+    *o++ = '.';
+    x = fp;
+    for (int k = prec - 1; k >= 0; k--) {
+        o[k] = (char)('0' + x % 10);
+        x /= 10;
+    }
+    o += prec;
+    *o = 0;
+    return (int)(o - out);
+}
+
+/* Parses n doubles from text as successive fscanf("%lf") calls would (strtod, the same
+ * conversion): returns the number of values missing at the end (set to 0), or -1 when
+ * a token is not entirely a number (the caller then reads the file with fscanf). */
+typedef struct parse_ctx {
+    const char *text;
+    long len;
+    const long *cut; /* chunk boundaries, at whitespace */
+    long *count;     /* tokens per chunk */
+    const long *first;
+    double *out;
+    long n;
+    int bad;
+} parse_ctx;
+
+static void parse_count(long lo, long hi, void *arg)
+{
+    parse_ctx *c = (parse_ctx *)arg;
+    for (long k = lo; k < hi; k++) {
+        long cnt = 0;
+        for (long i = c->cut[k]; i < c->cut[k + 1];) {
+            while (i < c->cut[k + 1] && isspace((unsigned char)c->text[i])) i++;
+            if (i >= c->cut[k + 1]) break;
+            cnt++;
+            while (i < c->cut[k + 1] && !isspace((unsigned char)c->text[i])) i++;
+        }
+        c->count[k] = cnt;
+    }
+}
+
+static void parse_values(long lo, long hi, void *arg)
+{
+    parse_ctx *c = (parse_ctx *)arg;
+    for (long k = lo; k < hi; k++) {
+        long idx = c->first[k];
+        for (long i = c->cut[k]; i < c->cut[k + 1] && idx < c->n;) {
+            while (i < c->cut[k + 1] && isspace((unsigned char)c->text[i])) i++;
+            if (i >= c->cut[k + 1]) break;
+            long e = i;
+            while (e < c->cut[k + 1] && !isspace((unsigned char)c->text[e])) e++;
+            char tok[128];
+            const long L = e - i;
+            if (L >= (long)sizeof tok) {
+                c->bad = 1;
+                return;
+            }
+            memcpy(tok, c->text + i, (size_t)L);
+            tok[L] = 0;
+            char *end;
+            c->out[idx++] = strtod(tok, &end);
+            if (end != tok + L) {
+                c->bad = 1;
+                return;
+            }
+            i = e;
+        }
+    }
+}
+
+long swehr_io_parse_doubles(const char *text, long len, double *out, long n)
+{
+    const int T = 64;
+    long cut[65], count[64], first[64];
+    cut[0] = 0;
+    for (int k = 1; k < T; k++) {
+        long c = len * k / T;
+        if (c < cut[k - 1]) c = cut[k - 1];
+        while (c < len && !isspace((unsigned char)text[c])) c++;
+        cut[k] = c;
+    }
+    cut[T] = len;
+    parse_ctx ctx = {text, len, cut, count, first, out, n, 0};
+    par_for(T, parse_count, &ctx);
+    long tot = 0;
+    for (int k = 0; k < T; k++) {
+        first[k] = tot;
+        tot += count[k];
+    }
+    par_for(T, parse_values, &ctx);
+    if (ctx.bad) return -1;
+    for (long i = tot; i < n; i++) out[i] = 0;
+    return tot < n ? n - tot : 0;
+}
+
+/* Reads the rest of f into memory (NUL-terminated); NULL on failure. */
+static char *slurp(FILE *f, long *len)
+{
+    size_t cap = 1 << 20, used = 0;
+    char *b = (char *)malloc(cap + 1);
+    if (!b) return NULL;
+    for (;;) {
+        const size_t got = fread(b + used, 1, cap - used, f);
+        used += got;
+        if (used < cap) break;
+        cap *= 2;
+        char *nb = (char *)realloc(b, cap + 1);
+        if (!nb) {
+            free(b);
+            return NULL;
+        }
+        b = nb;
+    }
+    b[used] = 0;
+    *len = (long)used;
+    return b;
+}
+
 static FILE *open_in(const char *dir, const char *name, char *path, size_t plen)
 {
     snprintf(path, plen, "%s/%s", dir, name);
@@ -36,7 +262,7 @@ static FILE *open_in(const char *dir, const char *name, char *path, size_t plen)
 }
 
 /* fscanf-style read of n doubles; missing values stay 0 and are counted */
-static int read_doubles(FILE *f, double *v, long n)
+static int read_doubles_scanf(FILE *f, double *v, long n)
 {
     int missing = 0;
     for (long k = 0; k < n; k++) {
@@ -46,6 +272,26 @@ static int read_doubles(FILE *f, double *v, long n)
         }
     }
     return missing;
+}
+
+/* The same values as read_doubles_scanf; large inputs (rasters) are parsed in parallel
+ * from memory, falling back to fscanf when a token is not a plain number. */
+static int read_doubles(FILE *f, double *v, long n)
+{
+    if (!f || n < 4096) return read_doubles_scanf(f, v, n);
+    const long pos = ftell(f);
+    long len = 0;
+    char *text = slurp(f, &len);
+    if (text) {
+        const long miss = swehr_io_parse_doubles(text, len, v, n);
+        free(text);
+        if (miss >= 0) return (int)miss;
+    }
+    if (pos < 0 || fseek(f, pos, SEEK_SET) != 0) {
+        for (long k = 0; k < n; k++) v[k] = 0;
+        return (int)n;
+    }
+    return read_doubles_scanf(f, v, n);
 }
 
 static char *trim(char *s)
@@ -91,6 +337,7 @@ static int read_sidecar(const char *dir, swehr_inputs *in, char *err, size_t err
         else if (!strcmp(key, "roughness")) p->roughness = atof(val);
         else if (!strcmp(key, "vinf0")) p->vinf0 = atof(val);
         else if (!strcmp(key, "movies")) in->movies = atoi(val);
+        else if (!strcmp(key, "discharge")) in->discharge = atoi(val);
         else if (!strcmp(key, "precision")) {
             if (!strcmp(val, "fp64")) in->precision = SWEHR_FP64;
             else if (!strcmp(val, "fp32")) in->precision = SWEHR_FP32;
@@ -239,29 +486,91 @@ void swehr_io_print_progress(FILE *out, double t, double courant, double dt, dou
     fprintf(out, "%lf\t %lf\t %lf\t %lf\n", t, courant, dt, rain);
 }
 
-int swehr_io_write_raster(FILE *f, const double *v, int nx, int ny)
+/* Raster rows j0 <= j < j0 + nb formatted into per-row buffers (the writers below). */
+typedef struct raster_ctx {
+    const double *v, *vy; /* vy != NULL: the speed raster pow(v^2 + vy^2, 0.5) */
+    int nx, ny, j0;
+    char **row;
+    size_t *cap;
+    long *rowlen;
+    int fail;
+} raster_ctx;
+
+static void format_rows(long lo, long hi, void *arg)
 {
-    for (int j = 1; j <= ny; j++)
-        for (int i = 1; i <= nx; i++)
-            if (fprintf(f, i != nx ? "%10.10lf\t" : "%10.10lf\n", v[(long)(i - 1) * ny + (j - 1)]) < 0) return -1;
-    return 0;
+    raster_ctx *c = (raster_ctx *)arg;
+    for (long r = lo; r < hi; r++) {
+        const int j = c->j0 + (int)r;
+        size_t used = 0;
+        for (int i = 1; i <= c->nx; i++) {
+            if (c->cap[r] - used < SWEHR_FIXED_MAX + 16) { /* room for the longest value */
+                const size_t ncap = 2 * c->cap[r] + SWEHR_FIXED_MAX;
+                char *nr = (char *)realloc(c->row[r], ncap);
+                if (!nr) {
+                    c->fail = 1;
+                    return;
+                }
+                c->row[r] = nr;
+                c->cap[r] = ncap;
+            }
+            const long k = (long)(i - 1) * c->ny + (j - 1);
+            const double x = c->vy ? pow(c->v[k] * c->v[k] + c->vy[k] * c->vy[k], 0.5) : c->v[k];
+            char *o = c->row[r] + used;
+            int L = swehr_io_format_fixed(o, x, 10);
+            if (L < 10) { /* "%10.10lf": only nan / inf are shorter than the width */
+                memmove(o + (10 - L), o, (size_t)L);
+                memset(o, ' ', (size_t)(10 - L));
+                L = 10;
+            }
+            o[L] = i != c->nx ? '\t' : '\n';
+            used += (size_t)L + 1;
+        }
+        c->rowlen[r] = (long)used;
+    }
 }
+
+static int write_rows(FILE *f, const double *v, const double *vy, int nx, int ny)
+{
+    enum { B = 256 }; /* rows per batch */
+    char *row[B];
+    size_t cap[B];
+    long rowlen[B];
+    int rc = 0;
+    for (int r = 0; r < B; r++) {
+        cap[r] = (size_t)nx * 24 + SWEHR_FIXED_MAX + 16;
+        row[r] = (char *)malloc(cap[r]);
+        if (!row[r]) rc = -1;
+    }
+    for (int j0 = 1; j0 <= ny && !rc; j0 += B) {
+        const int nb = ny - j0 + 1 < B ? ny - j0 + 1 : B;
+        raster_ctx c = {v, vy, nx, ny, j0, row, cap, rowlen, 0};
+        par_for(nb, format_rows, &c);
+        if (c.fail) rc = -1;
+        for (int r = 0; r < nb && !rc; r++)
+            if (fwrite(row[r], 1, (size_t)rowlen[r], f) != (size_t)rowlen[r]) rc = -1;
+    }
+    for (int r = 0; r < B; r++) free(row[r]);
+    return rc;
+}
+
+int swehr_io_write_raster(FILE *f, const double *v, int nx, int ny) { return write_rows(f, v, NULL, nx, ny); }
 
 int swehr_io_write_speed(FILE *f, const double *vx, const double *vy, int nx, int ny)
 {
-    for (int j = 1; j <= ny; j++)
-        for (int i = 1; i <= nx; i++) {
-            const long c = (long)(i - 1) * ny + (j - 1);
-            if (fprintf(f, i != nx ? "%10.10lf\t" : "%10.10lf\n", pow(vx[c] * vx[c] + vy[c] * vy[c], 0.5)) < 0) return -1;
-        }
-    return 0;
+    return write_rows(f, vx, vy, nx, ny);
 }
 
 int swehr_io_write_series(FILE *f, const double *cols, int ncols, int capacity, int count)
 {
-    for (int r = 0; r < count; r++)
-        for (int c = 0; c < ncols; c++)
-            if (fprintf(f, c != ncols - 1 ? "%lf\t" : "%lf\n", cols[(long)c * capacity + r]) < 0) return -1;
+    char line[(SWEHR_FIXED_MAX + 1) * 32];
+    for (int r = 0; r < count; r++) {
+        char *o = line;
+        for (int c = 0; c < ncols; c++) {
+            o += swehr_io_format_fixed(o, cols[(long)c * capacity + r], 6); /* "%lf" */
+            *o++ = c != ncols - 1 ? '\t' : '\n';
+        }
+        if (fwrite(line, 1, (size_t)(o - line), f) != (size_t)(o - line)) return -1;
+    }
     return 0;
 }
 
@@ -295,6 +604,27 @@ void swehr_io_close_movies(swehr_movies *m)
 {
     for (int k = 0; k < 5; k++)
         if (m->f[k]) fclose(m->f[k]);
+}
+
+int swehr_io_write_discharge(const char *dir, const double *t, const double *q, const double *qs, int count)
+{
+    char path[4096];
+    snprintf(path, sizeof path, "%s/discharge.txt", dir);
+    FILE *f = fopen(path, "w");
+    if (!f) return -1;
+    char line[3 * (SWEHR_FIXED_MAX + 1)];
+    int rc = 0;
+    for (int r = 0; r < count && !rc; r++) {
+        char *o = line;
+        o += swehr_io_format_fixed(o, t[r], 6);
+        *o++ = '\t';
+        o += swehr_io_format_fixed(o, q[r], 6);
+        *o++ = '\t';
+        o += swehr_io_format_fixed(o, qs[r], 6);
+        *o++ = '\n';
+        if (fwrite(line, 1, (size_t)(o - line), f) != (size_t)(o - line)) rc = -1;
+    }
+    return fclose(f) || rc ? -1 : 0;
 }
 
 int swehr_io_write_final(const char *dir, int nx, int ny, const double *const *fields, const double *stage,

@@ -28,6 +28,7 @@ typedef struct swehr_inputs {
     double *pfrac;  /* nclass fractions */
     double *rain;   /* rnum rates (m/s) */
     int movies;     /* 1: write movie frames at print intervals (reference behaviour) */
+    int discharge;  /* 1: also write discharge.txt (sidecar "discharge = 1"; not in the reference) */
     int precision;  /* enum swehr_precision requested by the sidecar */
     int short_reads;/* values missing from the input files (read as 0, like the reference) */
 } swehr_inputs;
@@ -51,6 +52,14 @@ int swehr_io_write_speed(FILE *f, const double *vx, const double *vy, int nx, in
  * col-major with stride capacity, "%lf" + tab / newline (SWEHR.c:1749-1761). */
 int swehr_io_write_series(FILE *f, const double *cols, int ncols, int capacity, int count);
 
+/* printf("%.<prec>f", v) into out (room for SWEHR_FIXED_MAX bytes), from the exact
+ * binary value with the same ties-to-even rounding; returns the length.  prec 1..18. */
+#define SWEHR_FIXED_MAX 400
+int swehr_io_format_fixed(char *out, double v, int prec);
+/* n doubles from text as successive fscanf("%lf") calls read them: returns the count of
+ * values missing at the end (set to 0), or -1 if a token is not entirely a number. */
+long swehr_io_parse_doubles(const char *text, long len, double *out, long n);
+
 /* Movie files, opened (and truncated) at start as the reference does (SWEHR.c:462-466). */
 typedef struct swehr_movies { FILE *f[5]; } swehr_movies;
 int swehr_io_open_movies(const char *dir, swehr_movies *m);
@@ -58,6 +67,10 @@ int swehr_io_open_movies(const char *dir, swehr_movies *m);
 int swehr_io_movie_frame(swehr_movies *m, const double *topo, const double *depth, const double *vx,
                          const double *vy, const double *totsed, const double *totm, int nx, int ny);
 void swehr_io_close_movies(swehr_movies *m);
+
+/* discharge.txt: one row per stage record, "%lf" time, outlet water (m^3/s) and
+ * sediment (kg/s) discharge, tab separated (an output the reference does not have). */
+int swehr_io_write_discharge(const char *dir, const double *t, const double *q, const double *qs, int count);
 
 /* Write the twelve final output files of SWEHR.c:1705-1761 into dir.  fields holds
  * i-major rasters indexed by enum swehr_field (CH = class 1); stage/saveflow as in

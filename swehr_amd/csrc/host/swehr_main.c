@@ -9,12 +9,32 @@
  * the five movie files and the reference's stdout lines.  Exit status 0 on success
  * (a CFL stop also ends normally, as in the reference), 1 on errors.
  */
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <time.h>
 
 #include "swehr.h"
 #include "swehr_io.h"
+
+/* Movie frames are written by a background thread from one of two host copies of the
+ * six frame fields, so the GPU resumes stepping as soon as a frame is downloaded. */
+enum { FRAME_FIELDS = 6 };
+static const int frame_ids[FRAME_FIELDS] = {SWEHR_F_TOPO, SWEHR_F_DEPTH, SWEHR_F_VELX,
+                                            SWEHR_F_VELY, SWEHR_F_TOTSED, SWEHR_F_TOTM};
+
+typedef struct frame_job {
+    swehr_movies *mv;
+    double *f[FRAME_FIELDS];
+    int nx, ny, rc;
+} frame_job;
+
+static void *write_frame(void *arg)
+{
+    frame_job *j = (frame_job *)arg;
+    j->rc = swehr_io_movie_frame(j->mv, j->f[0], j->f[1], j->f[2], j->f[3], j->f[4], j->f[5], j->nx, j->ny);
+    return NULL;
+}
 
 static int fail(const char *what, int rc)
 {
@@ -52,6 +72,19 @@ int main(int argc, char **argv)
         fld[f] = (double *)malloc(sizeof(double) * (size_t)n);
         if (!fld[f]) return fail("host allocation", SWEHR_ERR_OOM);
     }
+    frame_job jobs[2];
+    for (int b = 0; b < 2; b++) {
+        jobs[b].mv = &mv;
+        jobs[b].nx = nx;
+        jobs[b].ny = ny;
+        jobs[b].rc = 0;
+        for (int q = 0; q < FRAME_FIELDS; q++) {
+            jobs[b].f[q] = (double *)malloc(sizeof(double) * (size_t)n);
+            if (!jobs[b].f[q]) return fail("host allocation", SWEHR_ERR_OOM);
+        }
+    }
+    pthread_t writer;
+    int writing = 0, cur = 0;
     time_t t0 = time(NULL);
     swehr_step_info info;
     double t_before = 0;
@@ -66,15 +99,22 @@ int main(int argc, char **argv)
         }
         if (info.print_due) {
             swehr_io_print_progress(stdout, info.t, info.courant, info.dt, info.rain);
-            const int ids[6] = {SWEHR_F_TOPO, SWEHR_F_DEPTH, SWEHR_F_VELX, SWEHR_F_VELY, SWEHR_F_TOTSED, SWEHR_F_TOTM};
-            for (int q = 0; q < 6; q++)
-                if ((rc = swehr_download(h, ids[q], 0, fld[ids[q]])) != SWEHR_OK) return fail("download", rc);
-            swehr_io_movie_frame(&mv, fld[SWEHR_F_TOPO], fld[SWEHR_F_DEPTH], fld[SWEHR_F_VELX], fld[SWEHR_F_VELY],
-                                 fld[SWEHR_F_TOTSED], fld[SWEHR_F_TOTM], nx, ny);
+            /* the set written two frames ago is free: its writer was joined last frame */
+            for (int q = 0; q < FRAME_FIELDS; q++)
+                if ((rc = swehr_download(h, frame_ids[q], 0, jobs[cur].f[q])) != SWEHR_OK) return fail("download", rc);
             if ((rc = swehr_ack_print(h)) != SWEHR_OK) return fail("ack", rc);
+            if (writing) { /* frames are appended in order: one writer at a time */
+                pthread_join(writer, NULL);
+                writing = 0;
+            }
+            if (pthread_create(&writer, NULL, write_frame, &jobs[cur]) == 0) writing = 1;
+            else write_frame(&jobs[cur]);
+            cur ^= 1;
         }
         if (info.done) break;
     }
+    if (writing) pthread_join(writer, NULL);
+    if (jobs[0].rc || jobs[1].rc) fprintf(stderr, "swehr: writing a movie frame failed\n");
     time_t t1 = time(NULL);
     printf("Elapsed wall clock time: %ld\n", (long)(t1 - t0));
     for (int f = 0; f < SWEHR_F_COUNT; f++) {
@@ -93,7 +133,20 @@ int main(int argc, char **argv)
         fprintf(stderr, "swehr: cannot write outputs in %s\n", dir);
         return 1;
     }
+    if (in.discharge) { /* outlet hydrograph and sedigraph at the stage times */
+        double *q = (double *)malloc(sizeof(double) * (size_t)cap), *qs = (double *)malloc(sizeof(double) * (size_t)cap);
+        if (!q || !qs) return fail("host allocation", SWEHR_ERR_OOM);
+        if ((rc = swehr_discharge(h, q, qs, cap, &count)) != SWEHR_OK) return fail("discharge", rc);
+        if (swehr_io_write_discharge(dir, stage, q, qs, count < cap ? count : cap) != 0) {
+            fprintf(stderr, "swehr: cannot write discharge.txt in %s\n", dir);
+            return 1;
+        }
+        free(q);
+        free(qs);
+    }
     swehr_io_close_movies(&mv);
+    for (int b = 0; b < 2; b++)
+        for (int q = 0; q < FRAME_FIELDS; q++) free(jobs[b].f[q]);
     swehr_destroy(h);
     swehr_io_free(&in);
     return 0;

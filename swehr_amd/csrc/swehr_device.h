@@ -78,6 +78,9 @@ struct StepArgs {
     double *bud;       // [bud_slots][NB_TERMS][3]: running sum, compensation, last step's partial
     double *stage;     // [kStageCols][stage_cap]
     double *saveflow;  // [kFlowCols][stage_cap]
+    // outlet discharge: per column i the outflow through the j = 1/2 face in the last
+    // step (water m^3/s, sediment kg/s), and its sum at each stage record [2][stage_cap]
+    double *outq, *outs, *discharge;
 };
 
 template <typename T>
@@ -739,7 +742,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         const R hdt = R(0.5) * dt;
         double zn_c = s_z[zi * ZJ + zj];
         R totm = R(0), depn = R(0), chtemp = R(0);
-        double sb_out = 0;
+        double sb_out = 0, sq_out = 0;  // sediment leaving through all edges / through j = 1/2
         // class k from its suspended mass ch_k and deposited mass m_k
         auto update_class = [&](int k, R ch_k, R m_k) {
             const R *sk = s_sed + k * EXT;
@@ -747,7 +750,10 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             const R fkyl = fy0l * sk[uyl], fkyr = fy0r * sk[uyr];
             if (gi == 0) sb_out -= (double)fkxl;
             if (gi == nx - 1) sb_out += (double)fkxr;
-            if (gj == 0) sb_out -= (double)fkyl;
+            if (gj == 0) {
+                sb_out -= (double)fkyl;
+                sq_out -= (double)fkyl;
+            }
             if (gj == ny - 1) sb_out += (double)fkyr;
             R u = ch_k - dx_ * (fkxr - fkxl) - dx_ * (fkyr - fkyl);
             R m = m_k;
@@ -860,6 +866,10 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         if (gj == ny - 1) wb += (double)fy0r;
         bud[B_BWATER] = wb * dtd * A.dx;
         bud[B_BSED] = sb_out * dtd * A.dx;
+        if (gj == 0) {  // outlet discharge of this column (per unit width times dx)
+            gmem(A.outq)[gi] = -(double)fy0l * A.dx;
+            gmem(A.outs)[gi] = sq_out * A.dx;
+        }
         const double a2 = A.dx * A.dx;
         bud[B_EXCH] *= a2;
         bud[B_CLAMP] *= a2;
@@ -920,8 +930,8 @@ __device__ __forceinline__ void control_body(const StepArgs<R> &A)
 {
     glob<DevCtrl> *C = gmem(A.ctrl);
     if (!step_active(C)) return;
-    __shared__ double s_best[256];
-    __shared__ int s_idx[256];
+    __shared__ double s_best[16], s_q2[16];  // one per wave (blocks of up to 1024 threads)
+    __shared__ int s_idx[16];
     const long long steps = C->steps;
     const double t = C->t;
     const double smax = ctrl_smax(C);
@@ -962,6 +972,33 @@ __device__ __forceinline__ void control_body(const StepArgs<R> &A)
         __syncthreads();
         const int outlet = s_idx[0];
         const int r = C->stagecnt;
+        // outlet discharge of this step: the columns' outflows summed in a fixed order
+        {
+            double q = 0, qs = 0;
+            const glob<double> *oq = gmem(A.outq), *os = gmem(A.outs);
+            if (threadIdx.x < 256)  // stride 256 whatever the block size: the same order everywhere
+                for (int i = threadIdx.x; i < A.nx; i += 256) {
+                    q += oq[i];
+                    qs += os[i];
+                }
+            q = wave_sum(q);
+            qs = wave_sum(qs);
+            __syncthreads();  // s_best is reused
+            if (lane == 0) {
+                s_best[wid] = q;
+                s_q2[wid] = qs;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0 && r < cap) {
+                double tq = 0, tqs = 0;
+                for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+                    tq += s_best[w];
+                    tqs += s_q2[w];
+                }
+                gmem(A.discharge)[r] = tq;
+                gmem(A.discharge)[cap + r] = tqs;
+            }
+        }
         // one thread per stage column group: outlet state, its classes, the six probes
         // at (i = 2, j = 1, 10, ..., 50) (SWEHR.c:1605-1644)
         const int tid = threadIdx.x;
@@ -1027,7 +1064,7 @@ __device__ __forceinline__ void control_body(const StepArgs<R> &A)
 }
 
 template <typename R>
-__global__ __launch_bounds__(256) void control_kernel(StepArgs<R> A)
+__global__ __launch_bounds__(1024) void control_kernel(StepArgs<R> A)
 {
     control_body(A);
 }

@@ -291,6 +291,8 @@ size_t plan(swehr_handle *h, char *base, bool ks_raster)
     const size_t o_bud = L.add(sizeof(double) * (size_t)h->bud_slots * NB_TERMS * 3);
     const size_t o_stage = L.add(sizeof(double) * (size_t)kStageCols * h->stage_cap);
     const size_t o_flow = L.add(sizeof(double) * (size_t)kFlowCols * h->stage_cap);
+    const size_t o_outq = L.add(sizeof(double) * (size_t)h->nx), o_outs = L.add(sizeof(double) * (size_t)h->nx);
+    const size_t o_disc = L.add(sizeof(double) * 2 * (size_t)h->stage_cap);
     const size_t o_part = L.add(sizeof(double) * 10 * (size_t)h->budget_blocks);
     const size_t o_ctrl = L.add(sizeof(DevCtrl));
     if (base) {
@@ -313,6 +315,9 @@ size_t plan(swehr_handle *h, char *base, bool ks_raster)
         A.bud = (double *)(base + o_bud);
         A.stage = (double *)(base + o_stage);
         A.saveflow = (double *)(base + o_flow);
+        A.outq = (double *)(base + o_outq);
+        A.outs = (double *)(base + o_outs);
+        A.discharge = (double *)(base + o_disc);
         A.ctrl = (DevCtrl *)(base + o_ctrl);
         h->scratch = (double *)(base + o_scr);
         h->partials = (double *)(base + o_part);
@@ -407,7 +412,7 @@ int enqueue_step(swehr_handle *h, const StepArgs<R> &A)
 {
     hipLaunchKernelGGL((wave_speed_kernel<R>), dim3(h->k1_blocks), dim3(256), 0, h->stream, A);
     launch_step<R>(h->tile, 8 * h->blocks_per_xcd, h->lds, h->stream, A);
-    hipLaunchKernelGGL((control_kernel<R>), dim3(1), dim3(256), 0, h->stream, A);
+    hipLaunchKernelGGL((control_kernel<R>), dim3(1), dim3(1024), 0, h->stream, A);
     HIPCHK(hipGetLastError());
     return SWEHR_OK;
 }
@@ -710,7 +715,7 @@ static int profile_impl(swehr_handle *h, const StepArgs<R> &A, int nsteps, doubl
         HIPCHK(hipEventRecord(ev[1], h->stream));
         launch_step<R>(h->tile, 8 * h->blocks_per_xcd, h->lds, h->stream, A);
         HIPCHK(hipEventRecord(ev[2], h->stream));
-        hipLaunchKernelGGL((control_kernel<R>), dim3(1), dim3(256), 0, h->stream, A);
+        hipLaunchKernelGGL((control_kernel<R>), dim3(1), dim3(1024), 0, h->stream, A);
         HIPCHK(hipEventRecord(ev[3], h->stream));
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventSynchronize(ev[3]));
@@ -850,6 +855,24 @@ extern "C" int swehr_stage(swehr_handle *h, double *stage, double *saveflow, int
         if (saveflow)
             HIPCHK(hipMemcpy2DAsync(saveflow, sizeof(double) * capacity, df, sizeof(double) * h->stage_cap,
                                     sizeof(double) * n, kFlowCols, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+    }
+    if (count) *count = cnt;
+    return SWEHR_OK;
+}
+
+extern "C" int swehr_discharge(swehr_handle *h, double *q, double *qs, int capacity, int *count)
+{
+    if (!h || capacity < 0) return SWEHR_ERR_BAD_INPUT;
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipMemcpyAsync(h->hctrl, h->ctrl, sizeof(DevCtrl), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const int cnt = h->hctrl->stagecnt;
+    const int n = std::min(cnt, capacity);
+    const double *dd = h->precision == SWEHR_FP64 ? h->a64.discharge : h->a32.discharge;
+    if (n > 0) {
+        if (q) HIPCHK(hipMemcpyAsync(q, dd, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+        if (qs) HIPCHK(hipMemcpyAsync(qs, dd + h->stage_cap, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
     }
     if (count) *count = cnt;

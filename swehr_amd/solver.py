@@ -52,6 +52,7 @@ def lib():
     L.swehr_ack_print.argtypes = [C.c_void_p]
     L.swehr_download.argtypes = [C.c_void_p, C.c_int, C.c_int, P]
     L.swehr_stage.argtypes = [C.c_void_p, P, P, C.c_int, C.POINTER(C.c_int)]
+    L.swehr_discharge.argtypes = [C.c_void_p, P, P, C.c_int, C.POINTER(C.c_int)]
     L.swehr_budget.argtypes = [C.c_void_p, P]
     L.swehr_instance_bytes_for.restype = C.c_int64
     L.swehr_instance_bytes_for.argtypes = [C.POINTER(_abi.Params), C.c_int, C.c_int]
@@ -198,6 +199,17 @@ class Solver:
         check("swehr_stage", L.swehr_stage(self._h, dptr(st), dptr(fl), cap, C.byref(cnt)))
         n = cnt.value
         return st.reshape(_abi.STAGE_COLS, cap)[:, :n].copy(), fl.reshape(_abi.FLOW_COLS, cap)[:, :n].copy()
+
+    def discharge(self):
+        """(q, qs): outlet water (m^3/s) and sediment (kg/s) through the j = 1/2 face at
+        each stage record (swehr_discharge)."""
+        L = lib()
+        cnt = C.c_int(0)
+        check("swehr_discharge", L.swehr_discharge(self._h, None, None, 0, C.byref(cnt)))
+        cap = max(1, cnt.value)
+        q, qs = np.zeros(cap), np.zeros(cap)
+        check("swehr_discharge", L.swehr_discharge(self._h, dptr(q), dptr(qs), cap, C.byref(cnt)))
+        return q[:cnt.value].copy(), qs[:cnt.value].copy()
 
     def budget(self) -> np.ndarray:
         out = np.zeros(_abi.BUDGET_N)

@@ -34,6 +34,8 @@ def lib():
         L.oracle_download.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]
         L.oracle_stage.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int,
                                    C.POINTER(C.c_int)]
+        L.oracle_discharge.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int,
+                                       C.POINTER(C.c_int)]
         L.oracle_budget.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         L.oracle_fail_events.restype = C.c_long
         L.oracle_fail_events.argtypes = [C.c_void_p]
@@ -86,6 +88,14 @@ class Oracle:
         lib().oracle_stage(self._h, dptr(st), dptr(fl), cap, C.byref(cnt))
         n = cnt.value
         return st.reshape(_abi.STAGE_COLS, cap)[:, :n].copy(), fl.reshape(_abi.FLOW_COLS, cap)[:, :n].copy()
+
+    def discharge(self):
+        """(q, qs): outlet water (m^3/s) and sediment (kg/s) at each stage record."""
+        cap = _abi.STAGE_MAX
+        q, qs = np.zeros(cap), np.zeros(cap)
+        cnt = C.c_int(0)
+        lib().oracle_discharge(self._h, dptr(q), dptr(qs), cap, C.byref(cnt))
+        return q[:cnt.value].copy(), qs[:cnt.value].copy()
 
     def budget(self) -> np.ndarray:
         out = np.zeros(_abi.BUDGET_N)

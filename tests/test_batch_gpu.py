@@ -46,6 +46,10 @@ def _same(a, b, case):
     sb, fb = b.stage()
     np.testing.assert_array_equal(sa, sb)
     np.testing.assert_array_equal(fa, fb)
+    qa, qsa = a.discharge()
+    qb, qsb = b.discharge()
+    np.testing.assert_array_equal(qa, qb)
+    np.testing.assert_array_equal(qsa, qsb)
 
 
 @pytest.mark.parametrize("precision", [_abi.FP64, _abi.FP32], ids=["fp64", "fp32"])

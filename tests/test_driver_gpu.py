@@ -55,3 +55,29 @@ def test_driver_outputs_match_oracle_program(name, tmp_path):
         m = ~np.isnan(va)
         tol = 2e-6 if fn in ("stage.txt", "saveflow.txt") else 1e-8
         np.testing.assert_allclose(vb[m], va[m], rtol=1e-7, atol=tol, err_msg=fn)
+
+
+def test_driver_writes_outlet_discharge_on_request(tmp_path):
+    """Sidecar "discharge = 1" adds discharge.txt (time, outlet water and sediment
+    discharge at each stage record); the values are the library's series."""
+    from parity_util import outlet_case
+    from swehr_amd import solver as sv
+    from swehr_amd.inputs import write_case
+
+    case = outlet_case(tend=120.0)
+    d = str(tmp_path / "run")
+    os.makedirs(d)
+    write_case(case, d)
+    with open(os.path.join(d, "swehr_ext.txt"), "a") as fh:
+        fh.write("discharge = 1\n")
+    p = subprocess.run([DRIVER, d], capture_output=True, text=True, timeout=600)
+    assert p.returncode == 0, p.stderr
+    rows = np.loadtxt(os.path.join(d, "discharge.txt"))
+    with sv.Solver(case) as s:
+        s.run()
+        q, qs = s.discharge()
+        st, _ = s.stage()
+    assert rows.shape == (len(q), 3)
+    np.testing.assert_allclose(rows[:, 0], st[0], atol=1e-6)
+    np.testing.assert_allclose(rows[:, 1], q, rtol=0, atol=1e-6)
+    np.testing.assert_allclose(rows[:, 2], qs, rtol=0, atol=1e-6)

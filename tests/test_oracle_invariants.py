@@ -34,3 +34,27 @@ def test_oracle_thread_count_invariant():
     b.run(300)
     for k in ("depth", "uh", "vh", "topo", "totm"):
         np.testing.assert_array_equal(a.field(k), b.field(k))
+
+
+def test_outlet_discharge_integrates_to_boundary_budget_and_reaches_rain_rate():
+    """The oracle's outlet discharge series is pinned two ways.  (1) With dt = 1 s and a
+    stage record every step, the series summed over the run equals the net boundary
+    outflow of the compensated budget (the other three sides are walls, whose mass flux
+    is exactly zero): a wrong sign, face or width factor fails.  (2) Without
+    infiltration the outflow of a plane under constant rain tends to rain rate x area,
+    a closed form."""
+    from parity_util import outlet_case
+
+    case = outlet_case(tend=3000.0)
+    o = Oracle(case, threads=4)
+    o.run()
+    q, qs = o.discharge()
+    info = o.info()
+    b = o.budget()
+    o.close()
+    assert len(q) == info["steps"] == 3000
+    assert abs(q.sum() - b[3]) <= 1e-12 * abs(b[3])
+    assert abs(qs.sum() - b[7]) <= 1e-12 * max(abs(b[7]), 1e-30)
+    area_rate = 3e-5 * case.nx * case.ny * case.dx ** 2
+    assert abs(q[-1] / area_rate - 1) < 0.02
+    assert np.all(q >= -1e-12) and np.all(qs >= -1e-12)

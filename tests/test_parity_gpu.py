@@ -136,3 +136,29 @@ def test_fp32_mass_balance_large_grid():
     print(b, sv.water_residual(b) / wscale, sv.sediment_residual(b) / sscale)
     assert abs(sv.water_residual(b)) <= 1e-6 * wscale
     assert abs(sv.sediment_residual(b)) <= 1e-6 * sscale
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,tol", [(_abi.FP64, 1e-10), (_abi.FP32, 1e-4)])
+def test_outlet_discharge_matches_oracle(precision, tol):
+    """The outlet hydrograph and sedigraph (water and sediment through the j = 1/2 face at
+    each stage record) against the oracle's, and summed over a run with a record every
+    step against the GPU's own boundary-outflow budget."""
+    from parity_util import outlet_case
+
+    case = outlet_case(tend=400.0)
+    print("outlet_plane", case.nx, case.ny, "precision", precision, flush=True)
+    o = Oracle(case, threads=4)
+    o.run()
+    oq, oqs = o.discharge()
+    o.close()
+    with sv.Solver(case, precision=precision) as s:
+        s.run()
+        q, qs = s.discharge()
+        b = s.budget()
+        steps = s.info()["steps"]
+    assert len(q) == len(oq) == steps == 400
+    assert rel_l2(q, oq) < tol
+    assert rel_l2(qs, oqs) < tol
+    assert abs(q.sum() - b[3]) <= 1e-12 * abs(b[3]) * (1 if precision == _abi.FP64 else 1e4)
+    assert abs(qs.sum() - b[7]) <= 1e-12 * abs(b[7]) * (1 if precision == _abi.FP64 else 1e4)
