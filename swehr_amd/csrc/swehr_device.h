@@ -366,6 +366,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
 
     double *s_z = reinterpret_cast<double *>(smem_raw), *s_zp = s_z + ZEXT;
     R *s_h = reinterpret_cast<R *>(s_zp + ZEXT);
+    __shared__ double s_exp2[32];  // 2^(j/32) for the FP64 exponentials of phase 3
     R *s_u = s_h + EXT, *s_v = s_u + EXT, *s_ts = s_v + EXT, *s_dep = s_ts + EXT;
     R *s_fx = s_dep + EXT, *s_fy = s_fx + 3 * NFX;
     R *s_sed = s_fy + 3 * NFY;
@@ -374,6 +375,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     const R rhos = R(kRhos), rhow = R(kRhow);
     const R inv_rhos = R(1.0 / kRhos);
     const int tid = threadIdx.x;
+    if (sizeof(R) == sizeof(double) && tid < 32) s_exp2[tid] = kExp2Frac[tid];
 
     // Own-cell values of the update, loaded first so that their latency overlaps the
     // halo staging below (every load of a phase is issued before the first wait).
@@ -697,7 +699,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         if (active) {
             const R qq = hu * hu + hv * hv;
             const R qm = msqrt(qq);
-            const R sf = manning2_over_hpow<3>(h, rough) * qq;
+            const R sf = manning2_over_hpow<3>(h, rough, s_exp2) * qq;
             omega = rho * gz * sf * qm;
             const R oneH = R(1) - H;
             f1 = oneH * R(A.a) * Rr * shield;
@@ -708,6 +710,11 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             f3 = oneH * R(A.F) / R(A.J) * omega;
             if (e4ok) f4 = (R(1) - R(kBetaBedload)) * H * itm * R(A.F) * mrcp((rhos - rho) * inv_rhos * gz * h) * omega;
         }
+        // debris shutoff (SWEHR.c:1391-1399) zeroes every source and the settling: with
+        // the factors zeroed each class's E1..E4 are exactly 0 (E2, E4 are taken only
+        // where M_k is finite and positive), and vf is scaled by 0
+        const R vf_keep = shutoff ? R(0) : R(1);
+        if (shutoff) f1 = f2 = f3 = f4 = R(0);
 
         // conservative update (SWEHR.c:1424-1441)
         const int fxl = il * TJ + jl, fxr = fxl + TJ;
@@ -758,21 +765,20 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             R u = ch_k - dx_ * (fkxr - fkxl) - dx_ * (fkyr - fkyl);
             R m = m_k;
             // sources of class k from the state before this update (SWEHR.c:1322-1399)
+            // (outside active cells f1..f4 and omega are 0 and e2ok, e4ok false, so every
+            // E is 0 as the reference's guard makes it)
             const R dk = R(A.ds[k]), pk = R(A.pf[k]);
-            R e1 = R(0), e2 = R(0), e3 = R(0), e4 = R(0);
-            if (active) {
-                e1 = pk * f1;
-                if (e2ok && m > R(1e-8)) e2 = m * f2;
-                const bool ent = omega > R(0) && h > dk;
-                if (ent) e3 = pk * f3;
-                if (ent && e4ok && m > R(0)) e4 = m * f4;
-            }
+            const R e1 = pk * f1;
+            const R e2 = (e2ok && m > R(1e-8)) ? m * f2 : R(0);
+            const bool ent = omega > R(0) && h > dk;
+            const R e3 = ent ? pk * f3 : R(0);
+            const R e4 = (ent && e4ok && m > R(0)) ? m * f4 : R(0);
             const R ak = R(A.ak[k]);
             R vf = msqrt(R(A.ak2[k]) + cg * dk) - ak;
             vf = vf * hind;
             if (dk > R(0.0001) && shallow_v) vf = R(2e-3) + vshal * (vf - R(2e-3));
             if (dk > R(0.002) && shallow_v) vf = R(2e-3) + vshal * (vf - R(2e-3));
-            if (shutoff) { e1 = e2 = e3 = e4 = R(0); vf = R(0); }
+            vf = vf * vf_keep;
             // exchange with the deposited layer
             if (hn < R(10) * hfilm) {
                 m += dt * e1 + u;
@@ -781,7 +787,9 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             } else {
                 chtemp = u;
                 const R red = mfmin(m, dt * (e2 + e4));
-                const R decay = mexp(-vf * hdt * mrcp(hn));
+                R decay;
+                if constexpr (sizeof(R) == sizeof(double)) decay = exp_tab(-vf * hdt * mrcp(hn), s_exp2);
+                else decay = mexp(-vf * hdt * mrcp(hn));
                 u = decay * u;
                 u = u + dt * (e1 + e3) + red;
                 u = decay * u;
@@ -840,7 +848,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         // implicit friction (SWEHR.c:1553-1565) and maxima (SWEHR.c:1587-1588)
         R v1 = R(0), v2 = R(0);
         if (hn > hfilm) {
-            const R sf2 = manning2_over_hpow<2>(hn, rough) * gz * msqrt(hun * hun + hvn * hvn);
+            const R sf2 = manning2_over_hpow<2>(hn, rough, s_exp2) * gz * msqrt(hun * hun + hvn * hvn);
             const R den = mrcp(R(1) + dt * sf2);
             hun = hun * den;
             hvn = hvn * den;
@@ -1064,7 +1072,7 @@ __device__ __forceinline__ void control_body(const StepArgs<R> &A)
 }
 
 template <typename R>
-__global__ __launch_bounds__(1024) void control_kernel(StepArgs<R> A)
+__global__ __launch_bounds__(256) void control_kernel(StepArgs<R> A)
 {
     control_body(A);
 }

@@ -412,7 +412,7 @@ int enqueue_step(swehr_handle *h, const StepArgs<R> &A)
 {
     hipLaunchKernelGGL((wave_speed_kernel<R>), dim3(h->k1_blocks), dim3(256), 0, h->stream, A);
     launch_step<R>(h->tile, 8 * h->blocks_per_xcd, h->lds, h->stream, A);
-    hipLaunchKernelGGL((control_kernel<R>), dim3(1), dim3(1024), 0, h->stream, A);
+    hipLaunchKernelGGL((control_kernel<R>), dim3(1), dim3(256), 0, h->stream, A);
     HIPCHK(hipGetLastError());
     return SWEHR_OK;
 }
@@ -715,7 +715,7 @@ static int profile_impl(swehr_handle *h, const StepArgs<R> &A, int nsteps, doubl
         HIPCHK(hipEventRecord(ev[1], h->stream));
         launch_step<R>(h->tile, 8 * h->blocks_per_xcd, h->lds, h->stream, A);
         HIPCHK(hipEventRecord(ev[2], h->stream));
-        hipLaunchKernelGGL((control_kernel<R>), dim3(1), dim3(1024), 0, h->stream, A);
+        hipLaunchKernelGGL((control_kernel<R>), dim3(1), dim3(256), 0, h->stream, A);
         HIPCHK(hipEventRecord(ev[3], h->stream));
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventSynchronize(ev[3]));

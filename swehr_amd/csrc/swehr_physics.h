@@ -38,6 +38,40 @@ __device__ __forceinline__ float mfmin(float a, float b) { return __builtin_fmin
 __device__ __forceinline__ double mfmax(double a, double b) { return __builtin_fmax(a, b); }
 __device__ __forceinline__ float mfmax(float a, float b) { return __builtin_fmaxf(a, b); }
 __device__ __forceinline__ double mexp(double x) { return exp(x); }
+
+// 2^(j/32), j = 0..31, correctly rounded: the table of exp_tab (staged in LDS).
+__constant__ double kExp2Frac[32] = {
+    0x1.0000000000000p+0, 0x1.059b0d3158574p+0, 0x1.0b5586cf9890fp+0, 0x1.11301d0125b51p+0,
+    0x1.172b83c7d517bp+0, 0x1.1d4873168b9aap+0, 0x1.2387a6e756238p+0, 0x1.29e9df51fdee1p+0,
+    0x1.306fe0a31b715p+0, 0x1.371a7373aa9cbp+0, 0x1.3dea64c123422p+0, 0x1.44e086061892dp+0,
+    0x1.4bfdad5362a27p+0, 0x1.5342b569d4f82p+0, 0x1.5ab07dd485429p+0, 0x1.6247eb03a5585p+0,
+    0x1.6a09e667f3bcdp+0, 0x1.71f75e8ec5f74p+0, 0x1.7a11473eb0187p+0, 0x1.82589994cce13p+0,
+    0x1.8ace5422aa0dbp+0, 0x1.93737b0cdc5e5p+0, 0x1.9c49182a3f090p+0, 0x1.a5503b23e255dp+0,
+    0x1.ae89f995ad3adp+0, 0x1.b7f76f2fb5e47p+0, 0x1.c199bdd85529cp+0, 0x1.cb720dcef9069p+0,
+    0x1.d5818dcfba487p+0, 0x1.dfc97337b9b5fp+0, 0x1.ea4afa2a490dap+0, 0x1.f50765b6e4540p+0};
+
+// FP64 exp from the 2^(j/32) table tab (kExp2Frac in LDS): x = (32 m + j) ln2/32 + r with
+// |r| <= ln2/64 (ln2/32 split into a 32-bit head, so n * head is exact, and a tail);
+// exp(x) = 2^m 2^(j/32) exp(r), exp(r) by its degree-6 Taylor polynomial in nested form
+// 1 + r (1 + r/2 (... (1 + r/6))) (truncation < 4e-18; every addend is the inline 1).
+// Within 2 ulp of the correctly rounded value (checked against libm exp on 2e4
+// arguments) at about half the instructions of the library exp.  Below -746 the
+// result is 0 as exp's; NaN stays NaN.
+__device__ __forceinline__ double exp_tab(double x, const double *tab)
+{
+    x = x < -746.0 ? -746.0 : x;
+    const double n = __builtin_rint(x * 46.16624130844683);
+    double r = __builtin_fma(-n, 0x1.62e42fee00000p-6, x);
+    r = __builtin_fma(-n, 0x1.a39ef35793c76p-38, r);
+    const int ni = (int)n, j = ni & 31, m = ni >> 5;
+    double p = __builtin_fma(r, 1.0 / 6, 1.0);
+    p = __builtin_fma(r * 0.2, p, 1.0);
+    p = __builtin_fma(r * 0.25, p, 1.0);
+    p = __builtin_fma(r * (1.0 / 3), p, 1.0);
+    p = __builtin_fma(r * 0.5, p, 1.0);
+    p = __builtin_fma(r, p, 1.0);
+    return __builtin_ldexp(tab[j] * p, m);
+}
 __device__ __forceinline__ float mexp(float x) { return expf(x); }
 // 1/x for x finite and nonzero: the hardware estimate refined by two Newton steps in
 // FP64 (within 1 ulp, half the instructions of IEEE division); the 1-ulp hardware
@@ -125,14 +159,14 @@ constexpr double kLog2HcFriction = -8.380821783940931;
 // so its rounding costs ~1e-7 relative instead of the ~3e-6 of exp(-3.333 ln h) at
 // h = 1e-3; the hardware log2/exp2 (1 ulp) and reciprocal are then accurate enough.
 template <int PI, typename R>
-__device__ __forceinline__ R manning2_over_hpow(R h, R n0)
+__device__ __forceinline__ R manning2_over_hpow(R h, R n0, const double *exp_table)
 {
     if constexpr (sizeof(R) == sizeof(double)) {
         const double p = PI == 3 ? 3.333 : 2.333;
         const double lh = log(h);
         double ex = -p * lh;
         if (h <= kHcFriction) ex += 2 * kDepthExp * (lh - kLnHcFriction);
-        return n0 * n0 * exp(ex);
+        return n0 * n0 * exp_tab(ex, exp_table);
     } else {
         static_assert(PI == 2 || PI == 3, "friction powers 2.333 and 3.333");
         const float l2 = __builtin_amdgcn_logf(h);
