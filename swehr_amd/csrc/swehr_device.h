@@ -391,8 +391,19 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     double vinf_o = 0, ks_o = A.ks;
     const int lane = tid & 63, wid = tid >> 6;
     // ch_k and M_k of the first class (the class loop prefetches one class ahead)
-    R ch_a = R(0), m_a = R(0);
+    // ch_k of the own cell for the first KO classes, loaded with the halo staging (the
+    // same lines, so the requests merge in L2) and kept in registers for phase 3: read
+    // again there they would come from memory, the halo having left L2 by then
+    // (FP64 only: the FP32 kernel's register budget for 5 waves per SIMD has no room)
+    constexpr int KO = sizeof(R) == sizeof(double) ? 8 : 0;
+    R ch_own[KO > 0 ? KO : 1];
+#pragma unroll
+    for (int k = 0; k < KO; k++) ch_own[k] = R(0);
+    R m_a = R(0);
     if (own) {
+#pragma unroll
+        for (int k = 0; k < KO; k++)
+            if (k < S) ch_own[k] = at(chin + k * plane, cR);
         hu_o = at(huin, cR);
         hv_o = at(hvin, cR);
         totm_o = at(totmp, cR);
@@ -400,7 +411,6 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         md_o = at(maxdepthp, cR);
         vinf_o = at(vinfp, c8);
         if (A.ks_raster) ks_o = at(gmem(A.ks_raster), c8);
-        ch_a = at(chin, cR);
         m_a = at(Mp, cR);
 
     }
@@ -815,9 +825,18 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         // Two classes per trip with two register sets, each refilled for class k + 2
         // as soon as it is consumed: the loads of a class are in flight for two class
         // updates and no register copy waits on them.
-        // the next class's loads are issued before this class's update, so they are in
-        // flight for one class update
-        for (int k = 0; k < S; k++) {
+        // classes below KO: unrolled, ch_k from registers, M_{k+1} loaded before class k
+#pragma unroll
+        for (int k = 0; k < KO; k++) {
+            if (k < S) {
+                const R m_k = m_a;
+                if (k + 1 < S) m_a = at(Mp + (k + 1) * plane, cR);
+                update_class(k, ch_own[k], m_k);
+            }
+        }
+        // classes KO and up (S > KO): ch_k and M_k from memory, one class ahead
+        R ch_a = S > KO ? at(chin + KO * plane, cR) : R(0);
+        for (int k = KO; k < S; k++) {
             const R ch_k = ch_a, m_k = m_a;
             if (k + 1 < S) {
                 ch_a = at(chin + (k + 1) * plane, cR);
