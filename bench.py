@@ -134,6 +134,69 @@ def cpu_baseline(case, budget_s: float, threads: int):
     return 1 + more, el
 
 
+def parity_c1(threads: int):
+    """The reference's CPU-runnable configuration (100 x 40 plane, 1200 s) run in full by
+    the FP64 oracle on the host cores (timed: the CPU baseline at this size) and by the
+    GPU in FP64 and FP32, with the errors of each GPU result against the oracle's."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Oracle
+    from swehr_amd import _abi
+    from swehr_amd import solver as sv
+    from swehr_amd.inputs import config_c1
+
+    case = config_c1()
+    S, cells = case.nclass, case.nx * case.ny
+
+    def results(x):
+        st, _ = x.stage()
+        q, qs = x.discharge()
+        return {"depth": x.field("depth"), "speed": x.field("speed"),
+                "sed": [x.field("sed", k + 1) for k in range(S)], "stage_depth": st[1], "q": q, "qs": qs,
+                "budget": x.budget(), "steps": x.info()["steps"]}
+
+    o = Oracle(case, threads=threads)
+    t0 = time.perf_counter()
+    o.run()
+    cpu_s = time.perf_counter() - t0
+    ref = results(o)
+    o.close()
+    mid = copy.copy(case)
+    mid.tend = 600.0  # the end of the rain: the plane is running off
+    o = Oracle(mid, threads=threads)
+    o.run()
+    ref_mid = results(o)
+    o.close()
+    out = {"workload": "%dx%d plane, dx %.1f m, S=%d, %.0f s (%d steps), whole run" % (case.nx, case.ny, case.dx, S,
+                                                                                   case.tend, ref["steps"]),
+           "cpu_oracle": {"value": cells * S * ref["steps"] / cpu_s, "seconds": round(cpu_s, 3), "cores": threads},
+           "mass_balance_rel_oracle": dict(zip(("water", "sediment"), budget_errors(ref["budget"])))}
+    for name, prec in (("fp64", _abi.FP64), ("fp32", _abi.FP32)):
+        with sv.Solver(case, precision=prec) as s:
+            s.run()
+            g = results(s)
+        n = min(len(g["q"]), len(ref["q"]))
+        out[name] = {
+            "steps": g["steps"],
+            "rel_l2_depth": rel_l2(g["depth"], ref["depth"]),
+            "rel_l2_speed": rel_l2(g["speed"], ref["speed"]),
+            "rel_l2_concentration_per_class": [rel_l2(a, b) for a, b in zip(g["sed"], ref["sed"])],
+            "rel_l2_outlet_hydrograph": rel_l2(g["q"][:n], ref["q"][:n]),
+            "rel_l2_outlet_sedigraph": rel_l2(g["qs"][:n], ref["qs"][:n]),
+            "rel_l2_outlet_stage_depth": rel_l2(g["stage_depth"][:n], ref["stage_depth"][:n]),
+            "mass_balance_rel": dict(zip(("water", "sediment"), budget_errors(g["budget"]))),
+        }
+        with sv.Solver(mid, precision=prec) as s:
+            s.run()
+            gm = results(s)
+        out[name]["at_600s"] = {
+            "rel_l2_depth": rel_l2(gm["depth"], ref_mid["depth"]),
+            "rel_l2_speed": rel_l2(gm["speed"], ref_mid["speed"]),
+            "rel_l2_concentration_per_class": [rel_l2(a, b) for a, b in zip(gm["sed"], ref_mid["sed"])]}
+    return out
+
+
 def host_description() -> str:
     model = "unknown CPU"
     try:
@@ -247,6 +310,9 @@ def bench_batched(hip, count: int, window: int, scale_max: float):
     cells = sum(c.nx * c.ny for c in cases)
     S = cases[0].nclass
     warm = 20
+    import numpy as np
+
+    depth32, mb32 = [], []
     with sv.Batch(cases, precision=_abi.FP32) as b:
         best, tile_ms = b.autotune(3)
         b.advance(warm)
@@ -257,22 +323,47 @@ def bench_batched(hip, count: int, window: int, scale_max: float):
         hip.sync()
         batch_s = time.perf_counter() - t0
         steps = sum(b.member(i).info()["steps"] - warm for i in range(len(cases)))
+        for i in range(len(cases)):
+            m = b.member(i)
+            depth32.append(m.field("depth"))
+            mb32.append(budget_errors(m.budget()))
     batch_value = sum(c.nx * c.ny for c in cases) * S * window / batch_s
     plain_s = 0.0
     plain_cc = 0
-    for c in cases:
+    num = den = pnum = 0.0
+    worst_member = 0.0
+    for i, c in enumerate(cases):
         with sv.Solver(c, precision=_abi.FP64) as s:
             s.run(warm)
             n, ms, wall, _ = timed_run(hip, s, window)
             plain_s += wall
             plain_cc += c.nx * c.ny * S * n
+            d64 = s.field("depth")
+        # the same FP64 run with the topography perturbed by 1e-12 (relative): how much
+        # the FP64 depth itself moves under a perturbation far below FP32 rounding
+        pert = copy.copy(c)
+        pert.topo = c.topo * (1 + 1e-12 * np.random.default_rng(i).standard_normal(c.topo.shape))
+        with sv.Solver(pert, precision=_abi.FP64) as s:
+            s.run(warm + window)
+            dp = s.field("depth")
+        diff = float(np.sum((depth32[i] - d64) ** 2))
+        num += diff
+        pnum += float(np.sum((dp - d64) ** 2))
+        den += float(np.sum(d64 ** 2))
+        worst_member = max(worst_member, rel_l2(depth32[i], d64))
     plain_value = plain_cc / plain_s
     return {"workload": "%d hillslopes, %d cells total, S=%d, %d steps each after %d warm-up steps"
                         % (len(cases), cells, S, window, warm),
             "fp32_batched": {"value": batch_value, "wall_s": batch_s, "tile": best,
                              "autotune_ms_per_step": [round(float(x), 4) for x in tile_ms]},
             "fp64_plain_sequential": {"value": plain_value, "wall_s": plain_s},
-            "ratio": batch_value / plain_value, "member_steps": steps}
+            "ratio": batch_value / plain_value, "member_steps": steps,
+            "fp32_vs_fp64_after_the_window": {
+                "rel_l2_depth_all_members": (num / den) ** 0.5 if den > 0 else 0.0,
+                "rel_l2_depth_worst_member": worst_member,
+                "fp64_sensitivity_rel_l2_depth_all_members": (pnum / den) ** 0.5 if den > 0 else 0.0,
+                "mass_balance_rel_worst_member": {"water": max(x[0] for x in mb32),
+                                                  "sediment": max(x[1] for x in mb32)}}}
 
 
 def bench_serving(hip, nreq: int, rate: float, time_scale: float, mem_gib: float, slots: int, seed: int = 5):
@@ -421,7 +512,8 @@ def main():
                          "GPU on the same first 20 steps: %.4e cell-class-steps/s"
                          % (ncpu, args.grid, args.grid, el, cells * S * n_first / (ms_first * 1e-3)),
                "one_thread": {"value": cells * S * n1 / el1, "steps": n1, "seconds": round(el1, 2)},
-               "host": host_description()}
+               "host": host_description(),
+               "parity_c1": parity_c1(threads)}
         log("cpu baseline: %s" % cpu)
 
     forms = {}
