@@ -239,6 +239,9 @@ int swehr_batch_destroy(swehr_batch *b);
  * stay resident in HBM until released. */
 typedef struct swehr_server swehr_server;
 
+/* mem_bytes: the device-memory budget for resident states; 0 sizes it by the device's
+ * free memory at creation (15/16 of it).  An admission whose allocation still fails
+ * (the device is fuller than the budget says) leaves the request queued. */
 int swehr_server_create(int64_t mem_bytes, int max_slots, int precision, int tile, swehr_server **out);
 /* Queue a request (inputs are copied); *id identifies it.  SWEHR_ERR_BUSY if its
  * state can never fit the memory budget. */

@@ -477,8 +477,16 @@ struct swehr_server {
 
 extern "C" int swehr_server_create(int64_t mem_bytes, int max_slots, int precision, int tile, swehr_server **out)
 {
-    if (!out || mem_bytes <= 0 || max_slots < 1 || (precision != SWEHR_FP64 && precision != SWEHR_FP32))
+    if (!out || mem_bytes < 0 || max_slots < 1 || (precision != SWEHR_FP64 && precision != SWEHR_FP32))
         return SWEHR_ERR_BAD_INPUT;
+    if (mem_bytes == 0) {  // admit by the device's free memory: 15/16 of what is free now
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+            swehr_set_error("no device to size the server by%s%s", "", "");
+            return SWEHR_ERR_NO_DEVICE;
+        }
+        mem_bytes = (int64_t)(free_b / 16 * 15);
+    }
     swehr_server *s = new (std::nothrow) swehr_server();
     if (!s) return SWEHR_ERR_OOM;
     s->mem_budget = mem_bytes;

@@ -975,6 +975,7 @@ __device__ __forceinline__ void control_body(const StepArgs<R> &A)
         // outlet: deepest cell of row j = 1, first one on ties (SWEHR.c:1596-1604)
         double best = 0;
         int idx = 0x7fffffff;
+#pragma unroll 8
         for (int i = threadIdx.x; i < A.nx; i += blockDim.x) {
             const double hv = (double)gmem(A.h[q])[(long long)i * A.py];
             if (hv > best) { best = hv; idx = i; }
@@ -1004,6 +1005,7 @@ __device__ __forceinline__ void control_body(const StepArgs<R> &A)
             double q = 0, qs = 0;
             const glob<double> *oq = gmem(A.outq), *os = gmem(A.outs);
             if (threadIdx.x < 256)  // stride 256 whatever the block size: the same order everywhere
+#pragma unroll 8
                 for (int i = threadIdx.x; i < A.nx; i += 256) {
                     q += oq[i];
                     qs += os[i];

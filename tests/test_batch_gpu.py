@@ -167,3 +167,19 @@ def test_autotune_keeps_state_and_picks_a_fitting_tile():
         ref.run(200)
         for f in FIELDS:
             np.testing.assert_array_equal(s.field(f), ref.field(f))
+
+
+def test_server_sized_by_free_device_memory():
+    rng = np.random.default_rng(33)
+    c = config_c5_request(rng, 0, sizes=(64,), weights=(1,), tmin=30, tmax=40)
+    with sv.Server(mem_bytes=0, max_slots=2, precision=_abi.FP32) as srv:
+        used, budget = srv.stats()
+        assert used == 0 and budget > 8 * 2 ** 30  # most of the GPU's HBM is free
+        rid = srv.submit(c)
+        for _ in range(1000):
+            srv.pump(16)
+            d = srv.next_done()
+            if d is not None:
+                assert d[0] == rid
+                break
+        srv.release(rid)
