@@ -161,6 +161,14 @@ int swehr_download(swehr_handle *h, int field, int k, double *host);
  * saveflow[c*capacity + r] for c < SWEHR_FLOW_COLS, r < *count (SWEHR.c:1608-1675). */
 int swehr_stage(swehr_handle *h, double *stage, double *saveflow, int capacity, int *count);
 
+/* Per-cell parameters in place of scalars the reference hard-codes (it has no such
+ * rasters; SWEHR.c:237-242): suction head HF (m), theta_s - theta_0, and Manning's n0.
+ * values: nx*ny, i-major like topoin.txt, or NULL to return to the scalar.  Set on a
+ * single instance before stepping (refused for ensemble and server members); the
+ * rasters are constant, so snapshots do not hold them. */
+enum swehr_param_raster { SWEHR_R_HF = 0, SWEHR_R_DTHETA = 1, SWEHR_R_ROUGHNESS = 2, SWEHR_R_COUNT = 3 };
+int swehr_set_parameter_raster(swehr_handle *h, int which, const double *values);
+
 /* Outlet discharge at each stage record (the times of swehr_stage): the water (m^3/s)
  * and sediment (kg/s) leaving through the j = 1/2 face - the open side with the
  * reference's boundary types - in the step that ended at that time.  No counterpart in

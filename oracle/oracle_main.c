@@ -30,6 +30,7 @@ int main(int argc, char **argv)
     }
     swehr_io_print_setup(stdout, &in);
     oracle_t *o = oracle_create(&in.p, in.topo, in.ks, in.ds, in.pfrac, in.rain);
+    if (o && (in.hf || in.dtheta || in.roughness)) oracle_set_rasters(o, in.hf, in.dtheta, in.roughness);
     if (!o) {
         fprintf(stderr, "swehr_oracle_run: bad parameters\n");
         return 1;

@@ -60,6 +60,9 @@ struct oracle {
     double *sed, *totsed, *rho, *vel1, *vel2;
     /* parameters */
     double *ks, *ds, *pf, *rain;
+    /* per-cell suction head, theta_s - theta_0 and Manning n0 (the scalars of SWEHR.c:
+     * 237-242 unless a parameter raster is set; no raster in the reference) */
+    double *hfc, *dth, *n0c;
     /* step-transient */
     double *fx, *fy, *sx, *sy, *gmx, *gmy, *dfm, *btx, *bty, *gzn, *E, *vf;
     char *fail;
@@ -326,7 +329,8 @@ static void cell_sources(struct oracle *o, int i, int j)
     double mtstar = o->p.mtstar0;
     if (hvert > h0) mtstar = o->p.mtstar0 * pow(h0 / hvert, B_EXP);
     const double H = fmin(1, totm / mtstar);
-    const double manning = (h <= HCFRICTION) ? o->p.roughness * pow(h / HCFRICTION, DDEXP) : o->p.roughness;
+    const double n0 = o->n0c[c];
+    const double manning = (h <= HCFRICTION) ? n0 * pow(h / HCFRICTION, DDEXP) : n0;
     const double q = pow(o->hu[c] * o->hu[c] + o->hv[c] * o->hv[c], 0.5);
     const double sf = manning * manning * q * q / pow(h, 3.333);
     const double hindered = pow(1 - fmin(ts, 0.99), 4);
@@ -431,8 +435,8 @@ static void update_pass(struct oracle *o)
                         + dt * o->sy[c] + dt * o->gmy[c] + dt * o->dfm[c] * o->bty[c];
             /* Green-Ampt infiltration; negative when h < hfilm (film clamp), SWEHR.c:1448-1452 */
             const double ks = o->ks[c];
-            const double zf = o->vinf[c] / (o->p.thetas - o->p.theta0);
-            const double infl = fmin(h - HFILM, dt * ks * (zf + o->p.hf + h) / zf);
+            const double zf = o->vinf[c] / o->dth[c];
+            const double infl = fmin(h - HFILM, dt * ks * (zf + o->hfc[c] + h) / zf);
             o->vinf[c] += infl;
             h -= infl;
             double chk[SWEHR_MAX_CLASSES];
@@ -504,7 +508,8 @@ static void update_pass(struct oracle *o)
             }
             /* implicit friction (LeVeque), SWEHR.c:1553-1565 */
             if (h > HFILM) {
-                const double manning = (h <= HCFRICTION) ? o->p.roughness * pow(h / HCFRICTION, DDEXP) : o->p.roughness;
+                const double n0 = o->n0c[c];
+                const double manning = (h <= HCFRICTION) ? n0 * pow(h / HCFRICTION, DDEXP) : n0;
                 const double sf = manning * manning * o->gzn[c] * pow(hu * hu + hv * hv, 0.5) * 1 / pow(h, 2.333);
                 hu = hu / (1 + dt * sf);
                 hv = hv / (1 + dt * sf);
@@ -701,6 +706,7 @@ oracle_t *oracle_create(const swehr_params *p, const double *topo, const double 
     o->z = alloc0(P); o->zprev = alloc0(P); o->vinf = alloc0(P); o->totm = alloc0(P); o->dep = alloc0(P);
     o->maxvel = alloc0(P); o->maxdepth = alloc0(P); o->totsed = alloc0(P); o->rho = alloc0(P);
     o->vel1 = alloc0(P); o->vel2 = alloc0(P); o->ks = alloc0(P);
+    o->hfc = alloc0(P); o->dth = alloc0(P); o->n0c = alloc0(P);
     o->fx = alloc0((3 + S) * P); o->fy = alloc0((3 + S) * P);
     o->sx = alloc0(P); o->sy = alloc0(P); o->gmx = alloc0(P); o->gmy = alloc0(P);
     o->dfm = alloc0(P); o->btx = alloc0(P); o->bty = alloc0(P); o->gzn = alloc0(P);
@@ -725,6 +731,9 @@ oracle_t *oracle_create(const swehr_params *p, const double *topo, const double 
             o->vinf[c] = p->vinf0;
             o->rho[c] = RHOW;
             o->ks[c] = ks ? ks[n] : p->ks;
+            o->hfc[c] = p->hf;
+            o->dth[c] = p->thetas - p->theta0; /* the difference SWEHR.c:1449 forms */
+            o->n0c[c] = p->roughness;
             nsum(&zs, &zc, topo[n]);
         }
     o->z0sum = zs + zc;
@@ -772,6 +781,20 @@ int oracle_download(const oracle_t *o, int field, int k, double *host)
             default: return SWEHR_ERR_BAD_INPUT;
             }
             host[(long)(i - 1) * o->ny + (j - 1)] = v;
+        }
+    return SWEHR_OK;
+}
+
+int oracle_set_rasters(oracle_t *o, const double *hf, const double *dtheta, const double *roughness)
+{
+    if (!o) return SWEHR_ERR_BAD_INPUT;
+    for (int i = 1; i <= o->nx; i++)
+        for (int j = 1; j <= o->ny; j++) {
+            const long c = IX(i, j);
+            const long n = (long)(i - 1) * o->ny + (j - 1);
+            if (hf) o->hfc[c] = hf[n];
+            if (dtheta) o->dth[c] = dtheta[n];
+            if (roughness) o->n0c[c] = roughness[n];
         }
     return SWEHR_OK;
 }
@@ -829,7 +852,8 @@ void oracle_destroy(oracle_t *o)
 {
     if (!o) return;
     double *ptrs[] = {o->h, o->hu, o->hv, o->ch, o->M, o->sed, o->z, o->zprev, o->vinf, o->totm, o->dep,
-                      o->maxvel, o->maxdepth, o->totsed, o->rho, o->vel1, o->vel2, o->ks, o->fx, o->fy,
+                      o->maxvel, o->maxdepth, o->totsed, o->rho, o->vel1, o->vel2, o->ks, o->hfc, o->dth, o->n0c,
+                      o->fx, o->fy,
                       o->sx, o->sy, o->gmx, o->gmy, o->dfm, o->btx, o->bty, o->gzn, o->E, o->vf,
                       o->ds, o->pf, o->rain, o->stage, o->saveflow, o->discharge};
     for (size_t n = 0; n < sizeof(ptrs) / sizeof(ptrs[0]); n++) free(ptrs[n]);

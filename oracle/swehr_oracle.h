@@ -36,6 +36,9 @@ int oracle_download(const oracle_t *o, int field, int k, double *host);
 int oracle_stage(const oracle_t *o, double *stage, double *saveflow, int capacity, int *count);
 /* outlet discharge at each stage record: water (m^3/s) and sediment (kg/s) through the
  * j = 1/2 face in the step just taken */
+/* per-cell suction head (m), theta_s - theta_0 and Manning n0 in place of the scalars
+ * (each may be NULL: keep the scalar); nx*ny values, i-major; set before stepping */
+int oracle_set_rasters(oracle_t *o, const double *hf, const double *dtheta, const double *roughness);
 int oracle_discharge(const oracle_t *o, double *q, double *qs, int capacity, int *count);
 int oracle_budget(const oracle_t *o, double *out);
 long oracle_fail_events(const oracle_t *o);

@@ -342,22 +342,28 @@ static int read_sidecar(const char *dir, swehr_inputs *in, char *err, size_t err
             if (!strcmp(val, "fp64")) in->precision = SWEHR_FP64;
             else if (!strcmp(val, "fp32")) in->precision = SWEHR_FP32;
             else ok = 0;
-        } else if (!strcmp(key, "ks_file")) {
+        } else if (!strcmp(key, "ks_file") || !strcmp(key, "hf_file") || !strcmp(key, "dtheta_file") ||
+                   !strcmp(key, "roughness_file")) {
+            /* a per-cell raster in topoin.txt's layout (nx*ny values, i-major) */
+            double **dst = !strcmp(key, "ks_file")       ? &in->ks
+                           : !strcmp(key, "hf_file")     ? &in->hf
+                           : !strcmp(key, "dtheta_file") ? &in->dtheta
+                                                         : &in->roughness;
             char kp[4096];
             FILE *kf = open_in(dir, val, kp, sizeof kp);
             if (!kf) kf = fopen(val, "r");
             if (!kf) {
-                snprintf(err, errlen, "swehr_ext.txt:%d: cannot open ks_file %s", lineno, val);
+                snprintf(err, errlen, "swehr_ext.txt:%d: cannot open %s %s", lineno, key, val);
                 fclose(f);
                 return SWEHR_ERR_BAD_INPUT;
             }
             const long n = (long)p->nx * p->ny;
-            free(in->ks);
-            in->ks = (double *)malloc(sizeof(double) * (size_t)n);
-            const int miss = read_doubles(kf, in->ks, n);
+            free(*dst);
+            *dst = (double *)malloc(sizeof(double) * (size_t)n);
+            const int miss = *dst ? read_doubles(kf, *dst, n) : (int)n;
             fclose(kf);
             if (miss) {
-                snprintf(err, errlen, "ks_file %s: %d of %ld values missing", val, miss, n);
+                snprintf(err, errlen, "%s %s: %d of %ld values missing", key, val, miss, n);
                 fclose(f);
                 return SWEHR_ERR_BAD_INPUT;
             }
@@ -442,6 +448,9 @@ void swehr_io_free(swehr_inputs *in)
 {
     free(in->topo);
     free(in->ks);
+    free(in->hf);
+    free(in->dtheta);
+    free(in->roughness);
     free(in->ds);
     free(in->pfrac);
     free(in->rain);

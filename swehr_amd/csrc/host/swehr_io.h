@@ -24,6 +24,7 @@ typedef struct swehr_inputs {
     swehr_params p;
     double *topo;   /* nx*ny, i-major */
     double *ks;     /* nx*ny raster (sidecar ks_file) or NULL for p.ks */
+    double *hf, *dtheta, *roughness; /* sidecar hf_file, dtheta_file, roughness_file or NULL */
     double *ds;     /* nclass diameters (m) */
     double *pfrac;  /* nclass fractions */
     double *rain;   /* rnum rates (m/s) */

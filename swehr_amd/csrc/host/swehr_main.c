@@ -65,6 +65,10 @@ int main(int argc, char **argv)
     if (rc != SWEHR_OK) return fail("create", rc);
     rc = swehr_configure(h, in.movies ? 1 : 0, 0, 0);
     if (rc != SWEHR_OK) return fail("configure", rc);
+    const double *rasters[SWEHR_R_COUNT] = {in.hf, in.dtheta, in.roughness};
+    for (int k = 0; k < SWEHR_R_COUNT; k++)
+        if (rasters[k] && (rc = swehr_set_parameter_raster(h, k, rasters[k])) != SWEHR_OK)
+            return fail("parameter raster", rc);
     const int nx = in.p.nx, ny = in.p.ny;
     const long n = (long)nx * ny;
     double *fld[SWEHR_F_COUNT];

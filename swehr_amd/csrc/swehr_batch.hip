@@ -261,6 +261,7 @@ int make_instance(const swehr_params *p, const double *topo, const double *ks, c
         swehr_destroy(h);
         return rc;
     }
+    h->member = true;
     *out = h;
     return SWEHR_OK;
 }

@@ -65,6 +65,8 @@ struct StepArgs {
     R *M, *totm, *maxvel, *maxdepth;
     double *vinf;
     const double *ks_raster;  // per-cell Ks (pitched like the state) or null
+    // per-cell suction head, theta_s - theta_0 and Manning n0 (null: the scalars)
+    const double *hf_raster, *dtheta_raster, *n_raster;
     const double *rain;
     double ks, dx, h0, mtstar0, a, ad, F, J, roughness, theta0, thetas, hf, cstable, maxstep, tend, rint, vinf0;
     int rnum, inflow, bndry[4];
@@ -695,7 +697,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         const R mtstar = deep_v ? R(A.mtstar0) * (h0 * ihv) : R(A.mtstar0);
         const R H = mfmin(R(1), totm_old * mrcp(mtstar));  // mtstar = 0: NaN, which fmin drops as inf/NaN
         const R shield = deep_v ? h0 * ihv : R(1);
-        const R rough = R(A.roughness);
+        const R rough = A.n_raster ? R(at(gmem(A.n_raster), c8)) : R(A.roughness);
         const bool active = h > R(10) * hfilm && ts < R(0.6);
         const R oneh = R(1) - mfmin(ts, R(0.99));
         const R hind = (oneh * oneh) * (oneh * oneh);
@@ -740,11 +742,13 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         {
             const double ks = ks_o;
             const double vin = vinf_o;
-            const double zf = vin * mrcp(A.thetas - A.theta0);
+            const double dth = A.dtheta_raster ? at(gmem(A.dtheta_raster), c8) : A.thetas - A.theta0;
+            const double hf_c = A.hf_raster ? at(gmem(A.hf_raster), c8) : A.hf;
+            const double zf = vin * mrcp(dth);
             const double hd = (double)hn;
             // the film depth as stored in this precision, so a dry cell stays exactly dry
             // zf = 0 gives NaN where division gives inf (or NaN): fmin returns h - hfilm for both
-            const double infl = mfmin(hd - (double)hfilm, dtd * ks * (zf + A.hf + hd) * mrcp(zf));
+            const double infl = mfmin(hd - (double)hfilm, dtd * ks * (zf + hf_c + hd) * mrcp(zf));
             at(vinfp, c8) = vin + infl;
             hn = R(hd - infl);
         }

@@ -925,11 +925,56 @@ extern "C" int swehr_get_stream(swehr_handle *h, void **stream)
     return SWEHR_OK;
 }
 
+extern "C" int swehr_set_parameter_raster(swehr_handle *h, int which, const double *values)
+{
+    if (!h || which < 0 || which >= SWEHR_R_COUNT) return SWEHR_ERR_BAD_INPUT;
+    if (h->member) {
+        swehr_set_error("parameter rasters are set on single instances, not ensemble or server members%s%s", "", "");
+        return SWEHR_ERR_BAD_INPUT;
+    }
+    const long n = (long)h->nx * h->ny;
+    if (values)
+        for (long k = 0; k < n; k++) {
+            const double v = values[k];
+            // theta_s - theta_0 divides (SWEHR.c:1449): finite and nonzero; the others finite
+            if (!std::isfinite(v) || (which == SWEHR_R_DTHETA && v == 0) || (which == SWEHR_R_ROUGHNESS && v < 0)) {
+                swehr_set_error("parameter raster %s: bad value%s", which == SWEHR_R_HF ? "hf"
+                                : which == SWEHR_R_DTHETA ? "dtheta" : "roughness", "");
+                return SWEHR_ERR_BAD_INPUT;
+            }
+        }
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (values) {
+        if (!h->prast[which] && hipMalloc((void **)&h->prast[which], sizeof(double) * (size_t)h->plane) != hipSuccess) {
+            h->prast[which] = nullptr;
+            swehr_set_error("no device memory for a parameter raster%s%s", "", "");
+            return SWEHR_ERR_OOM;
+        }
+        const int rc = upload_raster<double>(h, h->prast[which], values);
+        if (rc) return rc;
+    } else if (h->prast[which]) {
+        HIPCHK(hipFree(h->prast[which]));
+        h->prast[which] = nullptr;
+    }
+    const double *d = h->prast[which];
+    if (which == SWEHR_R_HF) h->a64.hf_raster = h->a32.hf_raster = d;
+    else if (which == SWEHR_R_DTHETA) h->a64.dtheta_raster = h->a32.dtheta_raster = d;
+    else h->a64.n_raster = h->a32.n_raster = d;
+    if (h->gexec) {  // the captured steps hold the old arguments
+        (void)hipGraphExecDestroy(h->gexec);
+        h->gexec = nullptr;
+    }
+    return SWEHR_OK;
+}
+
 extern "C" int swehr_destroy(swehr_handle *h)
 {
     if (!h) return SWEHR_OK;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (double *d : h->prast)
+        if (d) (void)hipFree(d);
     if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
     if (h->graph) (void)hipGraphDestroy(h->graph);
     if (h->mem) (void)hipFree(h->mem);

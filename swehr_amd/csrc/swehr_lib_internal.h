@@ -50,6 +50,8 @@ struct swehr_handle {
     int budget_blocks = 0;
     swehr::StepArgs<double> a64{};
     swehr::StepArgs<float> a32{};
+    double *prast[3] = {nullptr, nullptr, nullptr};  // parameter rasters (device), separate allocations
+    bool member = false;                              // owned by an ensemble or a server
 };
 
 namespace swehr {

@@ -64,6 +64,10 @@ class Case:
     mtstar0: float = 3.0
     J: float = 30.0
     ks_raster: Optional[np.ndarray] = None  # (nx, ny) m/s, or None for sidecar["ks"]
+    # per-cell suction head (m), theta_s - theta_0, Manning n0, or None for the scalars
+    hf_raster: Optional[np.ndarray] = None
+    dtheta_raster: Optional[np.ndarray] = None
+    roughness_raster: Optional[np.ndarray] = None
     sidecar: dict = dataclasses.field(default_factory=dict)
     movies: int = 1
     name: str = "case"
@@ -125,12 +129,16 @@ def write_case(case: Case, d: str) -> None:
                 lines.append("bndry = %d %d %d %d" % tuple(v))
         elif float(v) != float(ref):
             lines.append("%s = %s" % (k, _fmt17(v) if k not in ("inflow",) else "%d" % int(v)))
-    if case.ks_raster is not None:
-        ks = np.asarray(case.ks_raster, dtype=np.float64)
-        assert ks.shape == (case.nx, case.ny)
-        with open(os.path.join(d, "ks.txt"), "w") as fh:
-            fh.write("\t".join(_fmt17(v) for v in ks.reshape(-1)) + "\n")
-        lines.append("ks_file = ks.txt")
+    for attr, key, fn in (("ks_raster", "ks_file", "ks.txt"), ("hf_raster", "hf_file", "hf.txt"),
+                          ("dtheta_raster", "dtheta_file", "dtheta.txt"),
+                          ("roughness_raster", "roughness_file", "roughness.txt")):
+        r = getattr(case, attr)
+        if r is not None:
+            r = np.asarray(r, dtype=np.float64)
+            assert r.shape == (case.nx, case.ny)
+            with open(os.path.join(d, fn), "w") as fh:
+                fh.write("\t".join(_fmt17(v) for v in r.reshape(-1)) + "\n")
+            lines.append("%s = %s" % (key, fn))
     if not case.movies:
         lines.append("movies = 0")
     path = os.path.join(d, "swehr_ext.txt")
@@ -172,8 +180,8 @@ def read_case(d: str, name: str = "case") -> Case:
                 k, val = [s.strip() for s in line.split("=", 1)]
                 if k == "bndry":
                     case.sidecar[k] = tuple(int(x) for x in val.replace(",", " ").split())
-                elif k == "ks_file":
-                    case.ks_raster = padded(nums(val), nx * ny).reshape(nx, ny)
+                elif k in ("ks_file", "hf_file", "dtheta_file", "roughness_file"):
+                    setattr(case, k.replace("_file", "_raster"), padded(nums(val), nx * ny).reshape(nx, ny))
                 elif k == "movies":
                     case.movies = int(val)
                 elif k == "inflow":

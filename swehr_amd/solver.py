@@ -53,6 +53,7 @@ def lib():
     L.swehr_download.argtypes = [C.c_void_p, C.c_int, C.c_int, P]
     L.swehr_stage.argtypes = [C.c_void_p, P, P, C.c_int, C.POINTER(C.c_int)]
     L.swehr_discharge.argtypes = [C.c_void_p, P, P, C.c_int, C.POINTER(C.c_int)]
+    L.swehr_set_parameter_raster.argtypes = [C.c_void_p, C.c_int, P]
     L.swehr_budget.argtypes = [C.c_void_p, P]
     L.swehr_instance_bytes_for.restype = C.c_int64
     L.swehr_instance_bytes_for.argtypes = [C.POINTER(_abi.Params), C.c_int, C.c_int]
@@ -127,6 +128,16 @@ def tile_shapes():
     return out
 
 
+def parameter_rasters(case):
+    """(hf, dtheta, roughness) rasters of a case as contiguous float64 arrays or None, in
+    the order of enum swehr_param_raster."""
+    out = []
+    for attr in ("hf_raster", "dtheta_raster", "roughness_raster"):
+        r = getattr(case, attr, None)
+        out.append(None if r is None else np.ascontiguousarray(r, dtype=np.float64).reshape(-1))
+    return out
+
+
 class Solver:
     """One GPU instance of a ``swehr_amd.inputs.Case``."""
 
@@ -148,6 +159,9 @@ class Solver:
         self._h = h
         check("swehr_configure", L.swehr_configure(self._h, 1 if pause_on_print else 0, int(steps_per_graph),
                                                    int(tile)))
+        for which, r in enumerate(parameter_rasters(case)):
+            if r is not None:
+                check("swehr_set_parameter_raster", L.swehr_set_parameter_raster(self._h, which, dptr(r)))
 
     def configure(self, pause_on_print: int = -1, steps_per_graph: int = 0, tile: int = 0):
         check("swehr_configure", lib().swehr_configure(self._h, int(pause_on_print), int(steps_per_graph), int(tile)))
@@ -291,6 +305,8 @@ class Batch:
     shape, per-instance dt)."""
 
     def __init__(self, cases, precision: int = _abi.FP64, tile: int = 0):
+        if any(r is not None for c in cases for r in parameter_rasters(c)):
+            raise ValueError("parameter rasters (hf, dtheta, roughness) are supported on single instances")
         L = lib()
         self.cases = list(cases)
         self.precision = precision
@@ -355,6 +371,8 @@ class Server:
         self.cases = {}
 
     def submit(self, case) -> int:
+        if any(r is not None for r in parameter_rasters(case)):
+            raise ValueError("parameter rasters (hf, dtheta, roughness) are supported on single instances")
         p = _abi.params_from_case(case)
         topo, ks, ds, pf, rain = _abi.case_arrays(case)
         rid = C.c_int64(0)

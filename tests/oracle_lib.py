@@ -34,6 +34,7 @@ def lib():
         L.oracle_download.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]
         L.oracle_stage.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int,
                                    C.POINTER(C.c_int)]
+        L.oracle_set_rasters.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 3
         L.oracle_discharge.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int,
                                        C.POINTER(C.c_int)]
         L.oracle_budget.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
@@ -60,6 +61,12 @@ class Oracle:
         if not h:
             raise ValueError("oracle_create rejected the case")
         self._h = h
+        from swehr_amd.solver import parameter_rasters
+
+        pr = parameter_rasters(case)
+        if any(r is not None for r in pr):
+            self._keep = self._keep + tuple(pr)
+            L.oracle_set_rasters(h, *[dptr(r) if r is not None else None for r in pr])
 
     def step(self) -> int:
         pd = C.c_int(0)

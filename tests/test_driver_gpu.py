@@ -81,3 +81,22 @@ def test_driver_writes_outlet_discharge_on_request(tmp_path):
     np.testing.assert_allclose(rows[:, 0], st[0], atol=1e-6)
     np.testing.assert_allclose(rows[:, 1], q, rtol=0, atol=1e-6)
     np.testing.assert_allclose(rows[:, 2], qs, rtol=0, atol=1e-6)
+
+
+def test_driver_reads_parameter_raster_files(tmp_path):
+    """hf_file, dtheta_file and roughness_file in the sidecar: the drop-in program and
+    the oracle's program read them alike and agree to the print precision."""
+    from swehr_amd.inputs import write_case
+    from test_oracle_invariants import _raster_case
+
+    case = _raster_case()
+    case.tend = 120.0
+    src = str(tmp_path / "in")
+    os.makedirs(src)
+    write_case(case, src)
+    assert os.path.exists(os.path.join(src, "dtheta.txt"))
+    _run(RUN_PATH, src, str(tmp_path / "oracle"))
+    _run(DRIVER, src, str(tmp_path / "gpu"))
+    for fn in ("depth.txt", "uh.txt", "vh.txt", "ch.txt"):
+        va, vb = _numbers(str(tmp_path / "oracle" / fn)), _numbers(str(tmp_path / "gpu" / fn))
+        np.testing.assert_allclose(vb, va, rtol=1e-7, atol=1e-8, err_msg=fn)

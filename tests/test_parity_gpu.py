@@ -197,3 +197,22 @@ def test_fp64_many_classes_match_oracle(S, nx, ny):
     assert rel_l2(q, oq) <= RTOL_FP64_SHORT and rel_l2(qs, oqs) <= RTOL_FP64_SHORT
     s.close()
     o.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,tol", [(_abi.FP64, 1e-10), (_abi.FP32, 1e-3)])
+def test_parameter_rasters_match_oracle(precision, tol):
+    """Per-cell suction head, theta_s - theta_0 and Manning n0 rasters (with a Ks
+    raster-free scalar Ks): the GPU state against the oracle's with the same rasters."""
+    from test_oracle_invariants import _raster_case
+
+    case = _raster_case()
+    print("raster case", case.nx, case.ny, "precision", precision, flush=True)
+    o = Oracle(case, threads=4)
+    o.run(300)
+    with sv.Solver(case, precision=precision) as s:
+        s.run(300)
+        errs = compare_states(s.state(), o.state(), case.nclass)
+    o.close()
+    k, e = worst(errs)
+    assert e <= tol, (k, e)
