@@ -66,6 +66,7 @@ struct Engine {
     size_t cap_members = 0, cap_map = 0;
     int n_map = 0, per_xcd = 0;
     size_t lds = 0;
+    int max_classes = 0;  // largest S of the members
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int graph_steps = 16;
@@ -152,6 +153,7 @@ struct Engine {
         std::vector<int> insts(n);
         std::vector<DevCtrl *> ctrls(n);
         lds = 0;
+        max_classes = 0;
         for (size_t i = 0; i < n; i++) {
             swehr_handle *h = members[i];
             if (precision == SWEHR_FP64) memcpy(&args[i * arg_size()], &h->a64, arg_size());
@@ -160,6 +162,7 @@ struct Engine {
             insts[i] = (int)i;
             ctrls[i] = h->ctrl;
             lds = std::max(lds, tile_lds_bytes(tile, precision, h->S));
+            max_classes = std::max(max_classes, h->S);
         }
         n_map = (int)nmap;
         per_xcd = (n_map + 7) / 8;
@@ -183,7 +186,7 @@ struct Engine {
         const int blocks = 8 * per_xcd;
         for (int s = 0; s < steps; s++) {
             launch_batch_wave(tile, precision, blocks, stream, d_args, d_map, n_map, per_xcd);
-            launch_batch_step(tile, precision, blocks, lds, stream, d_args, d_map, n_map, per_xcd);
+            launch_batch_step(tile, precision, blocks, lds, stream, d_args, d_map, n_map, per_xcd, max_classes);
             if (precision == SWEHR_FP64)
                 hipLaunchKernelGGL((batch_control_kernel<double>), dim3(n), dim3(256), 0, stream,
                                    (const StepArgs<double> *)d_args, (const int *)d_insts, n);

@@ -314,8 +314,17 @@ __device__ __forceinline__ void wave_speed_tile(const StepArgs<R> &A, int tile)
 
 // Waves per SIMD the step kernels are compiled for (VGPR budget 512 / waves): the
 // LDS of a tile admits 3 blocks per CU in FP64 and 5 in FP32 at the default tile.
-template <typename R>
+template <typename R, bool SPLIT = false>
 constexpr int kStepWaves = sizeof(R) == 4 ? 5 : 3;
+
+// Size classes whose upwind concentrations have LDS planes of their own (the first
+// chunk); in FP64 the classes above share the planes of the velocities, TOTSED, DEP
+// and face fluxes, which are dead by the time the class loop reaches them.  The FP64
+// tile's LDS is then that of 7 classes for any S, which keeps 3 blocks per CU.  FP32
+// keeps every class in its own planes: its second-chunk variant needs more registers
+// than 5 waves per SIMD allow and measured slower than the LDS-limited single chunk.
+template <typename R>
+constexpr int kLdsClasses = sizeof(R) == sizeof(double) ? 7 : kMaxClasses;
 
 template <typename R, int TI, int TJ>
 struct TileGeom {
@@ -323,9 +332,13 @@ struct TileGeom {
     static constexpr int EI = TI + 2, EJ = TJ + 2, EXT = EI * EJ;  // 1-cell halo
     static constexpr int ZI = TI + 4, ZJ = TJ + 4, ZEXT = ZI * ZJ;  // 2-cell halo (bed)
     static constexpr int NFX = (TI + 1) * TJ, NFY = TI * (TJ + 1);
+    static constexpr int KS = kLdsClasses<R>;
+    // the second chunk (classes KS .. S-1) overlays everything from the u plane on
+    static_assert((4 + KS) * EXT + 3 * NFX + 3 * NFY >= (kMaxClasses - KS) * EXT, "second class chunk");
     static size_t lds_bytes(int S)
     {
-        return sizeof(double) * 2 * ZEXT + sizeof(R) * ((size_t)(5 + S) * EXT + 3 * NFX + 3 * NFY);
+        const int s0 = S < KS ? S : KS;
+        return sizeof(double) * 2 * ZEXT + sizeof(R) * ((size_t)(5 + s0) * EXT + 3 * NFX + 3 * NFY);
     }
 };
 
@@ -336,12 +349,13 @@ struct TileGeom {
 // reciprocal computed once and multiplied, 1/sqrt via rsqrt, c* = |d|/4 in the
 // Riemann solver, the class-invariant source factors hoisted out of the class loop);
 // those change results at the rounding level only.
-template <typename R, int TI, int TJ>
+// SPLIT: the variant for any S up to kMaxClasses; without it S <= kLdsClasses.
+template <typename R, int TI, int TJ, bool SPLIT>
 __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsigned char *smem_raw)
 {
     using T = TileGeom<R, TI, TJ>;
     constexpr int NT = T::NT, EJ = T::EJ, EXT = T::EXT, ZJ = T::ZJ, ZEXT = T::ZEXT;
-    constexpr int NFX = T::NFX, NFY = T::NFY;
+    constexpr int NFX = T::NFX, NFY = T::NFY, KS = T::KS;
     glob<DevCtrl> *C = gmem(A.ctrl);
     if (!step_active(C)) return;
 
@@ -389,6 +403,9 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     // shared offset register, with no per-access 64-bit address arithmetic
     const unsigned c = (unsigned)gi * (unsigned)py + (unsigned)gj;
     const unsigned cR = c * (unsigned)sizeof(R), c8 = c * 8u;  // byte offsets
+    // the same for loads of phase 3, which threads outside the domain take part in when
+    // the classes come in two chunks (they read cell 0 and store nothing)
+    const unsigned cRL = own ? cR : 0u, c8L = own ? c8 : 0u;
     R hu_o = R(0), hv_o = R(0), totm_o = R(0), mv_o = R(0), md_o = R(0);
     double vinf_o = 0, ks_o = A.ks;
     const int lane = tid & 63, wid = tid >> 6;
@@ -397,7 +414,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     // same lines, so the requests merge in L2) and kept in registers for phase 3: read
     // again there they would come from memory, the halo having left L2 by then
     // (FP64 only: the FP32 kernel's register budget for 5 waves per SIMD has no room)
-    constexpr int KO = sizeof(R) == sizeof(double) ? 8 : 0;
+    constexpr int KO = sizeof(R) == sizeof(double) ? KS : 0;
     R ch_own[KO > 0 ? KO : 1];
 #pragma unroll
     for (int k = 0; k < KO; k++) ch_own[k] = R(0);
@@ -526,7 +543,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
                         ts_r[r] += sd * inv_rhos;
                         if (sd < R(-1e-5)) sd = R(0);
                     }
-                    s_sed[(k0 + kk) * EXT + e] = sd;
+                    if (k0 + kk < KS) s_sed[(k0 + kk) * EXT + e] = sd;
                 }
             }
         }
@@ -617,9 +634,11 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     }
     __syncthreads();
 
-    // ---- phase 3: sources and update of the tile's own cells
+    // ---- phase 3: sources and update of the tile's own cells.  With more than KS
+    // classes every thread of the block takes part (the second chunk of classes is
+    // staged between two barriers); threads outside the domain store nothing.
     double bud[NB_TERMS] = {0, 0, 0, 0, 0};
-    if (own) {
+    if (own || (SPLIT && S > KS)) {
         const int e = (il + 1) * EJ + (jl + 1);
         const int zi = il + 2, zj = jl + 2;
         const R h = s_h[e], hu = hu_o, hv = hv_o;
@@ -697,7 +716,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         const R mtstar = deep_v ? R(A.mtstar0) * (h0 * ihv) : R(A.mtstar0);
         const R H = mfmin(R(1), totm_old * mrcp(mtstar));  // mtstar = 0: NaN, which fmin drops as inf/NaN
         const R shield = deep_v ? h0 * ihv : R(1);
-        const R rough = A.n_raster ? R(at(gmem(A.n_raster), c8)) : R(A.roughness);
+        const R rough = A.n_raster ? R(at(gmem(A.n_raster), c8L)) : R(A.roughness);
         const bool active = h > R(10) * hfilm && ts < R(0.6);
         const R oneh = R(1) - mfmin(ts, R(0.99));
         const R hind = (oneh * oneh) * (oneh * oneh);
@@ -742,14 +761,14 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         {
             const double ks = ks_o;
             const double vin = vinf_o;
-            const double dth = A.dtheta_raster ? at(gmem(A.dtheta_raster), c8) : A.thetas - A.theta0;
-            const double hf_c = A.hf_raster ? at(gmem(A.hf_raster), c8) : A.hf;
+            const double dth = A.dtheta_raster ? at(gmem(A.dtheta_raster), c8L) : A.thetas - A.theta0;
+            const double hf_c = A.hf_raster ? at(gmem(A.hf_raster), c8L) : A.hf;
             const double zf = vin * mrcp(dth);
             const double hd = (double)hn;
             // the film depth as stored in this precision, so a dry cell stays exactly dry
             // zf = 0 gives NaN where division gives inf (or NaN): fmin returns h - hfilm for both
             const double infl = mfmin(hd - (double)hfilm, dtd * ks * (zf + hf_c + hd) * mrcp(zf));
-            at(vinfp, c8) = vin + infl;
+            if (own) at(vinfp, c8) = vin + infl;
             hn = R(hd - infl);
         }
 
@@ -765,8 +784,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         R totm = R(0), depn = R(0), chtemp = R(0);
         double sb_out = 0, sq_out = 0;  // sediment leaving through all edges / through j = 1/2
         // class k from its suspended mass ch_k and deposited mass m_k
-        auto update_class = [&](int k, R ch_k, R m_k) {
-            const R *sk = s_sed + k * EXT;
+        auto update_class = [&](int k, R ch_k, R m_k, const R *sk) {
             const R fkxl = fx0l * sk[uxl], fkxr = fx0r * sk[uxr];
             const R fkyl = fy0l * sk[uyl], fkyr = fy0r * sk[uyr];
             if (gi == 0) sb_out -= (double)fkxl;
@@ -787,8 +805,18 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             const bool ent = omega > R(0) && h > dk;
             const R e3 = ent ? pk * f3 : R(0);
             const R e4 = (ent && e4ok && m > R(0)) ? m * f4 : R(0);
+            // settling velocity sqrt(ak^2 + cg d) - ak (SWEHR.c:435).  For fine classes
+            // cg d is ~1e-6 of ak^2 and the difference keeps only the last digits of the
+            // sqrt: FP64 evaluates it as written, FP32 as cg d / (sqrt(ak^2 + cg d) + ak),
+            // equal in real arithmetic and free of the cancellation
             const R ak = R(A.ak[k]);
-            R vf = msqrt(R(A.ak2[k]) + cg * dk) - ak;
+            R vf;
+            if constexpr (sizeof(R) == sizeof(double)) {
+                vf = msqrt(R(A.ak2[k]) + cg * dk) - ak;
+            } else {
+                const R b = cg * dk;
+                vf = b * mrcp(msqrt(R(A.ak2[k]) + b) + ak);
+            }
             vf = vf * hind;
             if (dk > R(0.0001) && shallow_v) vf = R(2e-3) + vshal * (vf - R(2e-3));
             if (dk > R(0.002) && shallow_v) vf = R(2e-3) + vshal * (vf - R(2e-3));
@@ -823,34 +851,96 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             bud[B_EXCH] += (double)dh;
             totm += m;
             depn += m * inv_rhos;
-            at(chout + k * plane, cR) = u;
-            at(Mp + k * plane, cR) = m;
+            if (own) {
+                at(chout + k * plane, cR) = u;
+                at(Mp + k * plane, cR) = m;
+            }
         };
-        // Two classes per trip with two register sets, each refilled for class k + 2
-        // as soon as it is consumed: the loads of a class are in flight for two class
-        // updates and no register copy waits on them.
-        // classes below KO: unrolled, ch_k from registers, M_{k+1} loaded before class k
+        // first chunk, classes 0 .. S0-1.  Below KO: unrolled, ch_k from registers,
+        // M_{k+1} loaded before class k; from KO on ch_k and M_k come from memory, one
+        // class ahead.
+        const int S0 = S < KS ? S : KS;
 #pragma unroll
         for (int k = 0; k < KO; k++) {
-            if (k < S) {
+            if (k < S0) {
                 const R m_k = m_a;
-                if (k + 1 < S) m_a = at(Mp + (k + 1) * plane, cR);
-                update_class(k, ch_own[k], m_k);
+                if (k + 1 < S) m_a = at(Mp + (k + 1) * plane, cRL);
+                update_class(k, ch_own[k], m_k, s_sed + k * EXT);
             }
         }
-        // classes KO and up (S > KO): ch_k and M_k from memory, one class ahead
-        R ch_a = S > KO ? at(chin + KO * plane, cR) : R(0);
-        for (int k = KO; k < S; k++) {
+        const int kf = KO < S0 ? KO : KS;  // first class whose ch_k is read below
+        R ch_a = kf < S ? at(chin + kf * plane, cRL) : R(0);
+        for (int k = KO; k < S0; k++) {
             const R ch_k = ch_a, m_k = m_a;
             if (k + 1 < S) {
-                ch_a = at(chin + (k + 1) * plane, cR);
-                m_a = at(Mp + (k + 1) * plane, cR);
+                ch_a = at(chin + (k + 1) * plane, cRL);
+                m_a = at(Mp + (k + 1) * plane, cRL);
             }
-            update_class(k, ch_k, m_k);
+            update_class(k, ch_k, m_k, s_sed + k * EXT);
+        }
+        // second chunk, classes KS .. S-1: their upwind concentrations are staged as in
+        // phase 1 over the planes from s_u on, which no thread reads after this point
+        if (SPLIT && S > KS) {
+            __syncthreads();
+            R *const s_sed1 = s_u;
+            constexpr int KB = 2;  // classes per batch of halo loads
+            for (int k0 = KS; k0 < S; k0 += KB) {
+                R chb[KB][EIT];
+                unsigned cb[EIT];
+#pragma unroll
+                for (int r = 0; r < EIT; r++) {
+                    const int e = tid + r * NT;
+                    cb[r] = 0;
+                    if (e < EXT) {
+                        const int ei = e / EJ, ej = e - (e / EJ) * EJ;
+                        const int ggi = i0 - 1 + ei, ggj = j0 - 1 + ej;
+                        const int ci = ggi < 0 ? 0 : (ggi >= nx ? nx - 1 : ggi);
+                        const int cj = ggj < 0 ? 0 : (ggj >= ny ? ny - 1 : ggj);
+                        cb[r] = ((unsigned)ci * (unsigned)py + (unsigned)cj) * (unsigned)sizeof(R);
+                    }
+                }
+#pragma unroll
+                for (int kk = 0; kk < KB; kk++)
+#pragma unroll
+                    for (int r = 0; r < EIT; r++)
+                        if (k0 + kk < S && tid + r * NT < EXT) chb[kk][r] = at(chin + (k0 + kk) * plane, cb[r]);
+#pragma unroll
+                for (int r = 0; r < EIT; r++) {
+                    const int e = tid + r * NT;
+                    if (e < EXT) {
+                        const int ggj = j0 - 1 + (e - (e / EJ) * EJ);
+                        const R hx = s_h[e];
+                        const bool ghost = ggj >= ny && A.inflow, wet = hx > hfilm;
+                        const R ih = wet ? mrcp(hx) : R(0);
+#pragma unroll
+                        for (int kk = 0; kk < KB; kk++) {
+                            if (k0 + kk < S) {
+                                R sd = R(0);
+                                if (ghost) {
+                                    sd = R(in.c) / hx;
+                                } else if (wet) {
+                                    sd = chb[kk][r] * ih;
+                                    if (sd < R(-1e-5)) sd = R(0);
+                                }
+                                s_sed1[(k0 + kk - KS) * EXT + e] = sd;
+                            }
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            for (int k = KS; k < S; k++) {
+                const R ch_k = ch_a, m_k = m_a;
+                if (k + 1 < S) {
+                    ch_a = at(chin + (k + 1) * plane, cRL);
+                    m_a = at(Mp + (k + 1) * plane, cRL);
+                }
+                update_class(k, ch_k, m_k, s_sed1 + (k - KS) * EXT);
+            }
         }
 
         // dry-cell clamp (SWEHR.c:1532-1551)
-        if (!(hn > hfilm)) {
+        if (!(hn > hfilm) && own) {
             bud[B_CLAMP] += (double)(hfilm - hn);
             hn = hfilm;
             hun = R(0);
@@ -880,14 +970,16 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             v2 = hvn * ihn;
         }
         const R speed = msqrt(v1 * v1 + v2 * v2);
-        if (speed > mv_o) at(maxvelp, cR) = speed;
-        if (hn > md_o) at(maxdepthp, cR) = hn;
-        at(hout, cR) = hn;
-        at(huout, cR) = hun;
-        at(hvout, cR) = hvn;
-        at(zout, c8) = zn_c;
-        at(totmp, cR) = totm;
-        at(depout, cR) = depn;
+        if (own) {
+            if (speed > mv_o) at(maxvelp, cR) = speed;
+            if (hn > md_o) at(maxdepthp, cR) = hn;
+            at(hout, cR) = hn;
+            at(huout, cR) = hun;
+            at(hvout, cR) = hvn;
+            at(zout, c8) = zn_c;
+            at(totmp, cR) = totm;
+            at(depout, cR) = depn;
+        }
 
         // water and sediment leaving through the domain edges this step
         double wb = 0;
@@ -897,7 +989,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         if (gj == ny - 1) wb += (double)fy0r;
         bud[B_BWATER] = wb * dtd * A.dx;
         bud[B_BSED] = sb_out * dtd * A.dx;
-        if (gj == 0) {  // outlet discharge of this column (per unit width times dx)
+        if (gj == 0 && own) {  // outlet discharge of this column (per unit width times dx)
             gmem(A.outq)[gi] = -(double)fy0l * A.dx;
             gmem(A.outs)[gi] = sq_out * A.dx;
         }
@@ -915,7 +1007,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         double mine = 0;
 #pragma unroll
         for (int k = 0; k < NB_TERMS; k++) {
-            double v = bud[k];
+            double v = own ? bud[k] : 0.0;
             if (__ballot(v != 0.0) != 0ull) v = wave_sum(v);  // most terms are zero in most tiles
             if (lane == k) mine = v;
         }
@@ -923,8 +1015,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     }
 }
 
-template <typename R, int TI, int TJ>
-__global__ __launch_bounds__(TI *TJ, kStepWaves<R>) void step_kernel(StepArgs<R> A)
+template <typename R, int TI, int TJ, bool SPLIT>
+__global__ __launch_bounds__(TI *TJ, (kStepWaves<R, SPLIT>)) void step_kernel(StepArgs<R> A)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     // XCD-aware tile order: consecutive block ids land on different XCDs, so give
@@ -932,7 +1024,7 @@ __global__ __launch_bounds__(TI *TJ, kStepWaves<R>) void step_kernel(StepArgs<R>
     const int b = blockIdx.x;
     const int tile = (b & 7) * A.blocks_per_xcd + (b >> 3);
     if (tile >= A.nblocks) return;
-    step_tile<R, TI, TJ>(A, tile, smem_raw);
+    step_tile<R, TI, TJ, SPLIT>(A, tile, smem_raw);
 }
 
 // --------------------------------------------------------------------------- K3
@@ -1122,8 +1214,8 @@ __global__ __launch_bounds__(TI *TJ) void batch_wave_kernel(const StepArgs<R> *a
     wave_speed_tile<R, TI, TJ>(args[r.inst], r.tile);
 }
 
-template <typename R, int TI, int TJ>
-__global__ __launch_bounds__(TI *TJ, kStepWaves<R>) void batch_step_kernel(const StepArgs<R> *__restrict__ args,
+template <typename R, int TI, int TJ, bool SPLIT>
+__global__ __launch_bounds__(TI *TJ, (kStepWaves<R, SPLIT>)) void batch_step_kernel(const StepArgs<R> *__restrict__ args,
                                                                            const TileRef *__restrict__ map, int n,
                                                                            int per_xcd)
 {
@@ -1132,7 +1224,7 @@ __global__ __launch_bounds__(TI *TJ, kStepWaves<R>) void batch_step_kernel(const
     const int k = (b & 7) * per_xcd + (b >> 3);
     if (k >= n) return;
     const TileRef r = map[k];
-    step_tile<R, TI, TJ>(args[r.inst], r.tile, smem_raw);
+    step_tile<R, TI, TJ, SPLIT>(args[r.inst], r.tile, smem_raw);
 }
 
 template <typename R>

@@ -58,15 +58,29 @@ size_t tile_lds(int id, int S)
     }
 }
 
+template <typename R, bool SPLIT>
+void launch_step_v(int id, int blocks, size_t lds, hipStream_t s, const StepArgs<R> &A)
+{
+    switch (id) {
+    case 1: hipLaunchKernelGGL((step_kernel<R, 8, 32, SPLIT>), dim3(blocks), dim3(256), lds, s, A); break;
+    case 2: hipLaunchKernelGGL((step_kernel<R, 4, 64, SPLIT>), dim3(blocks), dim3(256), lds, s, A); break;
+    case 3: hipLaunchKernelGGL((step_kernel<R, 16, 16, SPLIT>), dim3(blocks), dim3(256), lds, s, A); break;
+    default: hipLaunchKernelGGL((step_kernel<R, 4, 32, SPLIT>), dim3(blocks), dim3(128), lds, s, A); break;
+    }
+}
+
+// the variant that stages the classes in two chunks only where there are more
+// classes than LDS planes for them
 template <typename R>
 void launch_step(int id, int blocks, size_t lds, hipStream_t s, const StepArgs<R> &A)
 {
-    switch (id) {
-    case 1: hipLaunchKernelGGL((step_kernel<R, 8, 32>), dim3(blocks), dim3(256), lds, s, A); break;
-    case 2: hipLaunchKernelGGL((step_kernel<R, 4, 64>), dim3(blocks), dim3(256), lds, s, A); break;
-    case 3: hipLaunchKernelGGL((step_kernel<R, 16, 16>), dim3(blocks), dim3(256), lds, s, A); break;
-    default: hipLaunchKernelGGL((step_kernel<R, 4, 32>), dim3(blocks), dim3(128), lds, s, A); break;
+    if constexpr (kLdsClasses<R> < kMaxClasses) {
+        if (A.S > kLdsClasses<R>) {
+            launch_step_v<R, true>(id, blocks, lds, s, A);
+            return;
+        }
     }
+    launch_step_v<R, false>(id, blocks, lds, s, A);
 }
 
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
@@ -99,15 +113,15 @@ static void batch_wave_t(int id, int blocks, hipStream_t s, const StepArgs<R> *a
     }
 }
 
-template <typename R>
+template <typename R, bool SP>
 static void batch_step_t(int id, int blocks, size_t lds, hipStream_t s, const StepArgs<R> *a, const TileRef *map,
                          int n, int per)
 {
     switch (id) {
-    case 1: hipLaunchKernelGGL((batch_step_kernel<R, 8, 32>), dim3(blocks), dim3(256), lds, s, a, map, n, per); break;
-    case 2: hipLaunchKernelGGL((batch_step_kernel<R, 4, 64>), dim3(blocks), dim3(256), lds, s, a, map, n, per); break;
-    case 3: hipLaunchKernelGGL((batch_step_kernel<R, 16, 16>), dim3(blocks), dim3(256), lds, s, a, map, n, per); break;
-    default: hipLaunchKernelGGL((batch_step_kernel<R, 4, 32>), dim3(blocks), dim3(128), lds, s, a, map, n, per); break;
+    case 1: hipLaunchKernelGGL((batch_step_kernel<R, 8, 32, SP>), dim3(blocks), dim3(256), lds, s, a, map, n, per); break;
+    case 2: hipLaunchKernelGGL((batch_step_kernel<R, 4, 64, SP>), dim3(blocks), dim3(256), lds, s, a, map, n, per); break;
+    case 3: hipLaunchKernelGGL((batch_step_kernel<R, 16, 16, SP>), dim3(blocks), dim3(256), lds, s, a, map, n, per); break;
+    default: hipLaunchKernelGGL((batch_step_kernel<R, 4, 32, SP>), dim3(blocks), dim3(128), lds, s, a, map, n, per); break;
     }
 }
 
@@ -119,10 +133,16 @@ void launch_batch_wave(int id, int prec, int blocks, hipStream_t s, const void *
 }
 
 void launch_batch_step(int id, int prec, int blocks, size_t lds, hipStream_t s, const void *args, const TileRef *map,
-                       int n, int per_xcd)
+                       int n, int per_xcd, int max_classes)
 {
-    if (prec == SWEHR_FP64) batch_step_t(id, blocks, lds, s, (const StepArgs<double> *)args, map, n, per_xcd);
-    else batch_step_t(id, blocks, lds, s, (const StepArgs<float> *)args, map, n, per_xcd);
+    static_assert(kLdsClasses<float> == kMaxClasses, "FP32 has no two-chunk variant");
+    if (prec == SWEHR_FP64) {
+        const StepArgs<double> *a = (const StepArgs<double> *)args;
+        if (max_classes > kLdsClasses<double>) batch_step_t<double, true>(id, blocks, lds, s, a, map, n, per_xcd);
+        else batch_step_t<double, false>(id, blocks, lds, s, a, map, n, per_xcd);
+    } else {
+        batch_step_t<float, false>(id, blocks, lds, s, (const StepArgs<float> *)args, map, n, per_xcd);
+    }
 }
 
 }  // namespace swehr

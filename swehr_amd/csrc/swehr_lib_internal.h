@@ -65,6 +65,8 @@ constexpr size_t kLdsLimit = 64 * 1024;
 // Ensemble launches of K1 / K2 for tile shape id (one block per map entry).
 void launch_batch_wave(int id, int prec, int blocks, hipStream_t s, const void *args, const TileRef *map, int n,
                        int per_xcd);
+// max_classes: the largest S of the members (selects the FP64 two-chunk variant above
+// kLdsClasses<double>)
 void launch_batch_step(int id, int prec, int blocks, size_t lds, hipStream_t s, const void *args, const TileRef *map,
-                       int n, int per_xcd);
+                       int n, int per_xcd, int max_classes);
 }  // namespace swehr

@@ -54,3 +54,18 @@ def outlet_case(nx: int = 20, ny: int = 30, tend: float = 300.0, seed: int = 5):
                 ds=np.array([2e-5, 1e-4, 5e-4]), pfrac=np.array([0.3, 0.4, 0.3]), a=2000.0, ad=4000.0,
                 sidecar=dict(cstable=0.9, maxstep=1.0, inflow=0, ks=0.0, roughness=0.03), movies=0,
                 name="outlet_plane")
+
+
+def many_classes_case(S: int, nx: int, ny: int, seed: int):
+    """A rough plane with S size classes 0.002-2 mm (S up to the maximum of 16), rain
+    with rainsplash and entrainment, a grid whose sizes are not multiples of any tile,
+    free outflow at j = 1/2 and the sediment-laden inflow at j = ny."""
+    from swehr_amd.inputs import Case, log_spaced_classes, planar_rough
+
+    rng = np.random.default_rng(seed)
+    topo = planar_rough(nx, ny, 1.0, 0.08, 0.01, rng)
+    topo -= topo.min()
+    return Case(nx=nx, ny=ny, dx=1.0, tend=600.0, topo=topo, rain=np.array([4e-5] * 6 + [0.0] * 6), rint=60.0,
+                ds=log_spaced_classes(S), pfrac=rng.dirichlet(np.full(S, 2.0)), a=2000.0, ad=4000.0, F=0.01,
+                sidecar=dict(cstable=0.5, ks=2e-6, hf=0.1, theta0=0.1, thetas=0.4, roughness=0.03), movies=0,
+                name="classes%d_%dx%d" % (S, nx, ny))

@@ -71,6 +71,23 @@ def test_ensemble_members_equal_alone_runs(precision):
         assert len({b.member(i).info()["steps"] for i in range(len(cases))}) > 1
 
 
+def test_ensemble_mixed_class_counts_equal_alone_runs():
+    """FP64 members with more size classes than the step kernel has LDS planes for
+    (the launch then takes the two-chunk variant for every member) beside members with
+    fewer: each equals its own run alone, which takes the variant its S selects."""
+    from parity_util import many_classes_case
+
+    cases = [many_classes_case(10, 37, 53, seed=1), _golden("transmissive"), many_classes_case(16, 21, 70, seed=2),
+             _golden("closed_basin")]
+    with sv.Batch(cases, precision=_abi.FP64) as b:
+        b.advance(300)
+        for i, c in enumerate(cases):
+            with sv.Solver(c, precision=_abi.FP64) as s:
+                s.run(300)
+                assert s.info()["steps"] == b.member(i).info()["steps"], c.name
+                _same(b.member(i), s, c)
+
+
 def test_ensemble_fp64_matches_oracle():
     cases = [_golden("transmissive"), _golden("closed_basin")]
     with sv.Batch(cases, precision=_abi.FP64) as b:

@@ -164,26 +164,14 @@ def test_outlet_discharge_matches_oracle(precision, tol):
     assert abs(qs.sum() - b[7]) <= 1e-12 * abs(b[7]) * (1 if precision == _abi.FP64 else 1e4)
 
 
-def _many_classes_case(S: int, nx: int, ny: int, seed: int):
-    """A rough plane with S size classes 0.002-2 mm (S up to the maximum of 16), rain
-    with rainsplash and entrainment, a grid whose sizes are not multiples of any tile,
-    free outflow at j = 1/2 and the sediment-laden inflow at j = ny."""
-    from swehr_amd.inputs import Case, log_spaced_classes, planar_rough
-
-    rng = np.random.default_rng(seed)
-    topo = planar_rough(nx, ny, 1.0, 0.08, 0.01, rng)
-    topo -= topo.min()
-    return Case(nx=nx, ny=ny, dx=1.0, tend=600.0, topo=topo, rain=np.array([4e-5] * 6 + [0.0] * 6), rint=60.0,
-                ds=log_spaced_classes(S), pfrac=rng.dirichlet(np.full(S, 2.0)), a=2000.0, ad=4000.0, F=0.01,
-                sidecar=dict(cstable=0.5, ks=2e-6, hf=0.1, theta0=0.1, thetas=0.4, roughness=0.03), movies=0,
-                name="classes%d_%dx%d" % (S, nx, ny))
-
-
 @pytest.mark.parametrize("S,nx,ny", [(10, 37, 53), (16, 21, 70), (9, 8, 33)])
 def test_fp64_many_classes_match_oracle(S, nx, ny):
-    """More than 8 classes (the kernel keeps the first 8 classes of a cell in registers
-    and reads the rest from memory) and the maximum of 16, on ragged grids."""
-    case = _many_classes_case(S, nx, ny, seed=S)
+    """More than 7 classes and the maximum of 16, on ragged grids.  The FP64 kernel keeps
+    the first 7 classes of a cell in registers and LDS planes of their own; the rest are
+    read from memory and staged in a second chunk over LDS planes that are dead by then."""
+    from parity_util import many_classes_case
+
+    case = many_classes_case(S, nx, ny, seed=S)
     print("case", case.name, flush=True)
     o, s = _pair(case, 400)
     errs = compare_states(s.state(), o.state(), case.nclass)
@@ -195,6 +183,21 @@ def test_fp64_many_classes_match_oracle(S, nx, ny):
     q, qs = s.discharge()
     oq, oqs = o.discharge()
     assert rel_l2(q, oq) <= RTOL_FP64_SHORT and rel_l2(qs, oqs) <= RTOL_FP64_SHORT
+    s.close()
+    o.close()
+
+
+def test_fp32_sixteen_classes_close_to_oracle():
+    """FP32 form at the maximum of 16 classes (all of them in LDS planes of the tile)."""
+    from parity_util import many_classes_case
+
+    case = many_classes_case(16, 37, 53, seed=7)
+    o, s = _pair(case, 400, precision=_abi.FP32)
+    errs = compare_states(s.state(), o.state(), case.nclass)
+    print({k: "%.2e" % v for k, v in errs.items()})
+    assert errs["depth"] <= 1e-4
+    k, e = worst(errs)
+    assert e <= 1e-3, (k, e)
     s.close()
     o.close()
 
