@@ -76,6 +76,9 @@ struct StepArgs {
     double ds[kMaxClasses], pf[kMaxClasses];
     // per-class settling constants a_k = 13.95 nu / d_k and a_k^2 (SWEHR.c:435)
     double ak[kMaxClasses], ak2[kMaxClasses];
+    // bit k: d_k > 0.0001, resp. d_k > 0.002, compared in this precision (the
+    // shallow-flow settling rules, SWEHR.c:1385-1388)
+    unsigned coarse1, coarse2;
     DevCtrl *ctrl;
     double *bud;       // [bud_slots][NB_TERMS][3]: running sum, compensation, last step's partial
     double *stage;     // [kStageCols][stage_cap]
@@ -406,8 +409,9 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     // the same for loads of phase 3, which threads outside the domain take part in when
     // the classes come in two chunks (they read cell 0 and store nothing)
     const unsigned cRL = own ? cR : 0u, c8L = own ? c8 : 0u;
-    R hu_o = R(0), hv_o = R(0), totm_o = R(0), mv_o = R(0), md_o = R(0);
-    double vinf_o = 0, ks_o = A.ks;
+    // own-cell values: threads outside the domain read cell 0 (and never store)
+    R hu_o, hv_o, totm_o, mv_o, md_o;
+    double vinf_o, ks_o = A.ks;
     const int lane = tid & 63, wid = tid >> 6;
     // ch_k and M_k of the first class (the class loop prefetches one class ahead)
     // ch_k of the own cell for the first KO classes, loaded with the halo staging (the
@@ -417,22 +421,15 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     constexpr int KO = sizeof(R) == sizeof(double) ? KS : 0;
     R ch_own[KO > 0 ? KO : 1];
 #pragma unroll
-    for (int k = 0; k < KO; k++) ch_own[k] = R(0);
-    R m_a = R(0);
-    if (own) {
-#pragma unroll
-        for (int k = 0; k < KO; k++)
-            if (k < S) ch_own[k] = at(chin + k * plane, cR);
-        hu_o = at(huin, cR);
-        hv_o = at(hvin, cR);
-        totm_o = at(totmp, cR);
-        mv_o = at(maxvelp, cR);
-        md_o = at(maxdepthp, cR);
-        vinf_o = at(vinfp, c8);
-        if (A.ks_raster) ks_o = at(gmem(A.ks_raster), c8);
-        m_a = at(Mp, cR);
-
-    }
+    for (int k = 0; k < KO; k++) ch_own[k] = k < S ? at(chin + k * plane, cRL) : R(0);
+    hu_o = at(huin, cRL);
+    hv_o = at(hvin, cRL);
+    totm_o = at(totmp, cRL);
+    mv_o = at(maxvelp, cRL);
+    md_o = at(maxdepthp, cRL);
+    vinf_o = at(vinfp, c8L);
+    if (A.ks_raster) ks_o = at(gmem(A.ks_raster), c8L);
+    R m_a = at(Mp, cRL);
 
     // ---- phase 1: ghosts and the 1-cell halo of the conserved state (SWEHR.c:816-1035)
     constexpr int EIT = (EXT + NT - 1) / NT, ZIT = (ZEXT + NT - 1) / NT;
@@ -783,17 +780,20 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         double zn_c = s_z[zi * ZJ + zj];
         R totm = R(0), depn = R(0), chtemp = R(0);
         double sb_out = 0, sq_out = 0;  // sediment leaving through all edges / through j = 1/2
+        const bool edge = gi == 0 || gi == nx - 1 || gj == 0 || gj == ny - 1;
         // class k from its suspended mass ch_k and deposited mass m_k
         auto update_class = [&](int k, R ch_k, R m_k, const R *sk) {
             const R fkxl = fx0l * sk[uxl], fkxr = fx0r * sk[uxr];
             const R fkyl = fy0l * sk[uyl], fkyr = fy0r * sk[uyr];
-            if (gi == 0) sb_out -= (double)fkxl;
-            if (gi == nx - 1) sb_out += (double)fkxr;
-            if (gj == 0) {
-                sb_out -= (double)fkyl;
-                sq_out -= (double)fkyl;
+            if (edge) {  // a branch: most waves hold no cell on the domain's edge
+                if (gi == 0) sb_out -= (double)fkxl;
+                if (gi == nx - 1) sb_out += (double)fkxr;
+                if (gj == 0) {
+                    sb_out -= (double)fkyl;
+                    sq_out -= (double)fkyl;
+                }
+                if (gj == ny - 1) sb_out += (double)fkyr;
             }
-            if (gj == ny - 1) sb_out += (double)fkyr;
             R u = ch_k - dx_ * (fkxr - fkxl) - dx_ * (fkyr - fkyl);
             R m = m_k;
             // sources of class k from the state before this update (SWEHR.c:1322-1399)
@@ -818,8 +818,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
                 vf = b * mrcp(msqrt(R(A.ak2[k]) + b) + ak);
             }
             vf = vf * hind;
-            if (dk > R(0.0001) && shallow_v) vf = R(2e-3) + vshal * (vf - R(2e-3));
-            if (dk > R(0.002) && shallow_v) vf = R(2e-3) + vshal * (vf - R(2e-3));
+            if (((A.coarse1 >> k) & 1u) && shallow_v) vf = R(2e-3) + vshal * (vf - R(2e-3));
+            if (((A.coarse2 >> k) & 1u) && shallow_v) vf = R(2e-3) + vshal * (vf - R(2e-3));
             vf = vf * vf_keep;
             // exchange with the deposited layer
             if (hn < R(10) * hfilm) {

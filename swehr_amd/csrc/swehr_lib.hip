@@ -402,11 +402,14 @@ void fill_args(swehr_handle *h, StepArgs<R> &A, const double *ds, const double *
     A.blocks_per_xcd = h->blocks_per_xcd;
     A.stage_cap = h->stage_cap;
     A.bud_slots = h->bud_slots;
+    A.coarse1 = A.coarse2 = 0;
     for (int k = 0; k < kMaxClasses; k++) {
         A.ds[k] = k < h->S ? ds[k] : 1.0;
         A.pf[k] = k < h->S ? pf[k] : 0.0;
         A.ak[k] = 13.95 * kNu / A.ds[k];
         A.ak2[k] = A.ak[k] * A.ak[k];
+        if ((R)A.ds[k] > (R)0.0001) A.coarse1 |= 1u << k;
+        if ((R)A.ds[k] > (R)0.002) A.coarse2 |= 1u << k;
     }
 }
 
