@@ -267,6 +267,17 @@ int swehr_server_request(swehr_server *s, int64_t id, swehr_handle **h);
 int swehr_server_release(swehr_server *s, int64_t id);
 /* Device memory held by admitted requests and the budget (bytes). */
 int swehr_server_stats(swehr_server *s, int64_t *mem_used, int64_t *mem_budget);
+/* Evict a running request: its whole device state is checkpointed into pinned host
+ * memory (as swehr_snapshot), its device memory freed, and it joins the back of the
+ * queue; when admitted again it is restored and continues exactly where it stopped.
+ * No counterpart in the reference (it has no serving loop). */
+int swehr_server_evict(swehr_server *s, int64_t id);
+/* Time slicing: with quantum > 0 steps, a pump whose queue head cannot be admitted
+ * first evicts running requests that have taken at least quantum steps since their
+ * admission (the longest-running first) until the head fits; 0 (the default) never
+ * evicts by itself.  *evictions counts every eviction so far. */
+int swehr_server_set_quantum(swehr_server *s, int64_t quantum);
+int swehr_server_evictions(swehr_server *s, int64_t *evictions);
 int swehr_server_destroy(swehr_server *s);
 
 /* Last error message of the calling thread (empty when none). */

@@ -461,7 +461,15 @@ struct Request {
     swehr_params p;
     std::vector<double> topo, ks, ds, pf, rain;
     int64_t bytes;
+    void *snap = nullptr;  // pinned host checkpoint of an evicted request's device state
+    int64_t snap_bytes = 0;
 };
+
+void free_snapshot(Request &r)
+{
+    if (r.snap) (void)hipHostFree(r.snap);
+    r.snap = nullptr;
+}
 
 }  // namespace
 
@@ -477,7 +485,63 @@ struct swehr_server {
     std::map<int64_t, swehr_handle *> running, finished;
     std::map<swehr_handle *, int64_t> ids;
     std::deque<std::pair<int64_t, swehr_step_info>> done_fifo;
+    std::map<int64_t, Request> live;       // inputs of the running requests (for re-admission)
+    std::map<int64_t, int64_t> slice;      // steps taken since (re)admission
+    int64_t quantum = 0, evictions = 0;
 };
+
+namespace {
+
+// Checkpoint running request id into pinned host memory, free its instance and queue
+// it at the back.
+int evict_request(swehr_server *s, int64_t id)
+{
+    auto it = s->running.find(id);
+    if (it == s->running.end()) return SWEHR_ERR_BAD_INPUT;
+    swehr_handle *h = it->second;
+    Engine &E = s->eng;
+    if (E.stream) HIPCHK(hipStreamSynchronize(E.stream));
+    int64_t bytes = 0;
+    int rc = swehr_snapshot_bytes(h, &bytes);
+    if (rc) return rc;
+    void *snap = nullptr;
+    if (hipHostMalloc(&snap, (size_t)bytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        swehr_set_error("no pinned host memory for the evicted state%s%s", "", "");
+        return SWEHR_ERR_OOM;
+    }
+    rc = swehr_snapshot(h, snap);
+    if (rc) {
+        (void)hipHostFree(snap);
+        return rc;
+    }
+    for (size_t i = 0; i < E.members.size(); i++)
+        if (E.members[i] == h) {
+            E.members.erase(E.members.begin() + (long)i);
+            break;
+        }
+    E.dirty = true;
+    s->running.erase(it);
+    s->ids.erase(h);
+    s->slice.erase(id);
+    s->mem_used -= (int64_t)h->bytes;
+    swehr_destroy(h);
+    Request r = std::move(s->live[id]);
+    s->live.erase(id);
+    r.snap = snap;
+    r.snap_bytes = bytes;
+    s->queue.push_back(std::move(r));
+    s->evictions++;
+    return SWEHR_OK;
+}
+
+bool head_fits(const swehr_server *s)
+{
+    return !s->queue.empty() && (int)s->running.size() < s->max_slots &&
+           s->mem_used + s->queue.front().bytes <= s->mem_budget;
+}
+
+}  // namespace
 
 extern "C" int swehr_server_create(int64_t mem_bytes, int max_slots, int precision, int tile, swehr_server **out)
 {
@@ -549,20 +613,52 @@ extern "C" int swehr_server_pump(swehr_server *s, int steps, int *running, int *
 {
     if (!s || steps < 0) return SWEHR_ERR_BAD_INPUT;
     Engine &E = s->eng;
-    // admission: FIFO, while a slot is free and the memory budget holds the request
-    while (!s->queue.empty() && (int)s->running.size() < s->max_slots &&
-           s->mem_used + s->queue.front().bytes <= s->mem_budget) {
+    // time slicing: make room for the queue head by evicting requests that have had
+    // their quantum, the longest-running first
+    while (s->quantum > 0 && !s->queue.empty() && !head_fits(s)) {
+        int64_t victim = -1, most = -1;
+        for (const auto &kv : s->running) {
+            const int64_t taken = s->slice[kv.first];
+            if (taken >= s->quantum && taken > most) {
+                most = taken;
+                victim = kv.first;
+            }
+        }
+        if (victim < 0) break;
+        const int rc = evict_request(s, victim);
+        if (rc) return rc;
+    }
+    // admission: FIFO, while a slot is free and the memory budget holds the request;
+    // an evicted request is restored from its checkpoint
+    while (head_fits(s)) {
         Request &r = s->queue.front();
         swehr_handle *h = nullptr;
         int rc = make_instance(&r.p, r.topo.data(), r.ks.empty() ? nullptr : r.ks.data(), r.ds.data(), r.pf.data(),
                                r.rain.data(), s->precision, E.tile, &h);
         if (rc == SWEHR_ERR_OOM) break;  // the device is fuller than the budget says: retry later
         if (rc) return rc;
+        if (r.snap) {
+            int64_t bytes = 0;
+            rc = swehr_snapshot_bytes(h, &bytes);
+            if (!rc && bytes != r.snap_bytes) {
+                swehr_set_error("re-admitted request's state size differs from its checkpoint%s%s", "", "");
+                rc = SWEHR_ERR_BAD_INPUT;
+            }
+            if (!rc) rc = swehr_restore(h, r.snap);
+            if (rc) {
+                swehr_destroy(h);
+                return rc;
+            }
+            free_snapshot(r);
+        }
         s->mem_used += (int64_t)h->bytes;
         s->running[r.id] = h;
         s->ids[h] = r.id;
+        s->slice[r.id] = 0;
         E.members.push_back(h);
         E.dirty = true;
+        const int64_t id = r.id;
+        s->live[id] = std::move(r);
         s->queue.pop_front();
     }
     if (steps > 0 && !E.members.empty()) {
@@ -577,6 +673,7 @@ extern "C" int swehr_server_pump(swehr_server *s, int steps, int *running, int *
             int rc = E.run_graph();
             if (rc) return rc;
             taken += E.graph_steps;
+            for (swehr_handle *h : E.members) s->slice[s->ids[h]] += E.graph_steps;
             std::vector<swehr_handle *> keep;
             for (size_t i = 0; i < E.members.size(); i++) {
                 swehr_handle *h = E.members[i];
@@ -595,6 +692,8 @@ extern "C" int swehr_server_pump(swehr_server *s, int steps, int *running, int *
                     info.stage_count = c.stagecnt;
                     info.status = c.status;
                     s->running.erase(id);
+                    s->live.erase(id);
+                    s->slice.erase(id);
                     s->finished[id] = h;
                     s->done_fifo.push_back({id, info});
                 } else {
@@ -653,10 +752,31 @@ extern "C" int swehr_server_stats(swehr_server *s, int64_t *mem_used, int64_t *m
     return SWEHR_OK;
 }
 
+extern "C" int swehr_server_evict(swehr_server *s, int64_t id)
+{
+    if (!s) return SWEHR_ERR_BAD_INPUT;
+    return evict_request(s, id);
+}
+
+extern "C" int swehr_server_set_quantum(swehr_server *s, int64_t quantum)
+{
+    if (!s || quantum < 0) return SWEHR_ERR_BAD_INPUT;
+    s->quantum = quantum;
+    return SWEHR_OK;
+}
+
+extern "C" int swehr_server_evictions(swehr_server *s, int64_t *evictions)
+{
+    if (!s || !evictions) return SWEHR_ERR_BAD_INPUT;
+    *evictions = s->evictions;
+    return SWEHR_OK;
+}
+
 extern "C" int swehr_server_destroy(swehr_server *s)
 {
     if (!s) return SWEHR_OK;
     if (s->eng.stream) (void)hipStreamSynchronize(s->eng.stream);
+    for (Request &r : s->queue) free_snapshot(r);
     for (auto &kv : s->running) swehr_destroy(kv.second);
     for (auto &kv : s->finished) swehr_destroy(kv.second);
     s->eng.release();

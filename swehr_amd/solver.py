@@ -58,6 +58,9 @@ def lib():
     L.swehr_instance_bytes_for.restype = C.c_int64
     L.swehr_instance_bytes_for.argtypes = [C.POINTER(_abi.Params), C.c_int, C.c_int]
     L.swehr_server_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.swehr_server_evict.argtypes = [C.c_void_p, C.c_int64]
+    L.swehr_server_set_quantum.argtypes = [C.c_void_p, C.c_int64]
+    L.swehr_server_evictions.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
     L.swehr_instance_bytes.restype = C.c_int64
     L.swehr_instance_bytes.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
     L.swehr_destroy.argtypes = [C.c_void_p]
@@ -409,6 +412,19 @@ class Server:
     def release(self, rid: int):
         check("swehr_server_release", lib().swehr_server_release(self._s, int(rid)))
         self.cases.pop(rid, None)
+
+    def evict(self, rid: int):
+        """Checkpoint a running request to host memory and requeue it (it resumes exactly)."""
+        check("swehr_server_evict", lib().swehr_server_evict(self._s, int(rid)))
+
+    def set_quantum(self, steps: int):
+        """Time slicing: evict requests that had `steps` steps when the queue head waits (0: off)."""
+        check("swehr_server_set_quantum", lib().swehr_server_set_quantum(self._s, int(steps)))
+
+    def evictions(self) -> int:
+        n = C.c_int64(0)
+        check("swehr_server_evictions", lib().swehr_server_evictions(self._s, C.byref(n)))
+        return n.value
 
     def close(self):
         if getattr(self, "_s", None):

@@ -47,6 +47,8 @@ def test_strerror_and_bad_input():
     info = _abi.StepInfo()
     assert L.swehr_advance(None, 1, C.byref(info)) == _abi.ERR_BAD_INPUT
     assert L.swehr_download(None, 0, 0, None) == _abi.ERR_BAD_INPUT
+    assert L.swehr_server_evict(None, 1) == _abi.ERR_BAD_INPUT
+    assert L.swehr_server_set_quantum(None, 16) == _abi.ERR_BAD_INPUT
 
 
 @pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")

@@ -200,3 +200,62 @@ def test_server_sized_by_free_device_memory():
                 assert d[0] == rid
                 break
         srv.release(rid)
+
+
+def _drain(srv, ids, reqs_by_id, max_pumps=20000):
+    """Pump until every request finished; check each against its alone run and release it.
+    Returns the largest number of requests running at once."""
+    done, max_running = 0, 0
+    for _ in range(max_pumps):
+        running, queued, finished = srv.pump(16)
+        max_running = max(max_running, running)
+        while True:
+            d = srv.next_done()
+            if d is None:
+                break
+            rid, info = d
+            c = reqs_by_id[rid]
+            r = srv.result(rid)
+            with sv.Solver(c, precision=srv.precision) as s:
+                s.run()
+                assert s.info()["steps"] == info["steps"], c.name
+                _same(r, s, c)
+            srv.release(rid)
+            done += 1
+        if done == len(ids):
+            return max_running
+    raise AssertionError("requests did not finish")
+
+
+def test_server_evicted_request_resumes_identically():
+    """A running request checkpointed to host memory and requeued frees its device memory
+    and, re-admitted, finishes exactly as if it had never stopped."""
+    rng = np.random.default_rng(34)
+    reqs = [config_c5_request(rng, k, sizes=(48, 64), weights=(1, 1), tmin=60.0, tmax=90.0) for k in range(2)]
+    with sv.Server(mem_bytes=1 << 30, max_slots=4, precision=_abi.FP32) as srv:
+        ids = {srv.submit(c): c for c in reqs}
+        srv.pump(32)
+        used0, _ = srv.stats()
+        victim = min(ids)
+        srv.evict(victim)
+        used1, _ = srv.stats()
+        assert used1 < used0
+        assert srv.evictions() == 1
+        with pytest.raises(sv.SwehrError):
+            srv.evict(victim)  # no longer running
+        _drain(srv, ids, ids)
+
+
+def test_server_time_slicing_round_robins_requests_that_do_not_fit_together():
+    """A budget that holds one request at a time and a quantum of 32 steps: the requests
+    take turns (evicted and restored many times), and each result equals its alone run."""
+    rng = np.random.default_rng(35)
+    reqs = [config_c5_request(rng, k, sizes=(48,), weights=(1,), tmin=60.0, tmax=90.0) for k in range(3)]
+    need = max(sv.instance_bytes_for(c, _abi.FP32) for c in reqs)
+    with sv.Server(mem_bytes=int(1.5 * need), max_slots=4, precision=_abi.FP32) as srv:
+        srv.set_quantum(32)
+        ids = {srv.submit(c): c for c in reqs}
+        max_running = _drain(srv, ids, ids)
+        assert max_running == 1
+        assert srv.evictions() >= 2
+
