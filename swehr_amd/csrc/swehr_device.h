@@ -79,6 +79,10 @@ struct StepArgs {
     // bit k: d_k > 0.0001, resp. d_k > 0.002, compared in this precision (the
     // shallow-flow settling rules, SWEHR.c:1385-1388)
     unsigned coarse1, coarse2;
+    // quotients of parameters, divided on the host exactly as the step would (IEEE
+    // division on both sides): 1/dx, and 1/(6 h0) in this precision
+    double inv_dx;
+    R inv_6h0;
     DevCtrl *ctrl;
     double *bud;       // [bud_slots][NB_TERMS][3]: running sum, compensation, last step's partial
     double *stage;     // [kStageCols][stage_cap]
@@ -237,7 +241,7 @@ __device__ __forceinline__ R wave_speed_cell(const StepArgs<R> &A, const glob<R>
     // differences at the edges (SWEHR.c:1240-1277) then read the cell itself
     const unsigned im = i > 0 ? c - py : c, ip = i < nx - 1 ? c + py : c;
     const unsigned jm = j > 0 ? c - 1 : c, jp = j < ny - 1 ? c + 1 : c;
-    const double odx = 1.0 / A.dx;
+    const double odx = A.inv_dx;
     const double fx = (i != 0 && i != nx - 1) ? 0.5 * odx : odx;
     const double fy = (j != 0 && j != ny - 1) ? 0.5 * odx : odx;
     const double sxd = fx * (at(zp, ip * 8u) - at(zp, im * 8u));
@@ -574,7 +578,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     }
     __syncthreads();
 
-    const double odxd = 1.0 / A.dx;
+    const double odxd = A.inv_dx;
     const R odx = R(odxd);
     const R g = R(kG);
     // GZ of the previous step's flux pass at global cell (gi, gj) (SWEHR.c:1279, read
@@ -735,7 +739,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         const R hind = (oneh * oneh) * (oneh * oneh);
         const bool shutoff = h > R(9) * h0 && ts > R(kCthreshold);
         const bool shallow_v = h < R(9) * h0;
-        const R vshal = mfmax(h - R(3) * h0, R(0)) * (R(1) / (R(6) * h0));
+        const R vshal = mfmax(h - R(3) * h0, R(0)) * A.inv_6h0;
         const R cg = R(1.09) * R(kRhos / kRhow - 1) * gz;
         // rainsplash and stream-power factors; class k multiplies by P_k or M_k
         R f1 = R(0), f2 = R(0), f3 = R(0), f4 = R(0), omega = R(0);
@@ -828,10 +832,10 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             const R ak = R(A.ak[k]);
             R vf;
             if constexpr (sizeof(R) == sizeof(double)) {
-                vf = msqrt(R(A.ak2[k]) + cg * dk) - ak;
+                vf = msqrt_pos(R(A.ak2[k]) + cg * dk) - ak;
             } else {
                 const R b = cg * dk;
-                vf = b * mrcp(msqrt(R(A.ak2[k]) + b) + ak);
+                vf = b * mrcp(msqrt_pos(R(A.ak2[k]) + b) + ak);
             }
             vf = vf * hind;
             if (((A.coarse1 >> k) & 1u) && shallow_v) vf = R(2e-3) + vshal * (vf - R(2e-3));

@@ -389,6 +389,8 @@ void fill_args(swehr_handle *h, StepArgs<R> &A, const double *ds, const double *
     A.thetas = p.thetas;
     A.hf = p.hf;
     A.cstable = p.cstable;
+    A.inv_dx = 1.0 / p.dx;
+    A.inv_6h0 = (R)1 / ((R)6 * (R)p.h0);
     A.maxstep = p.maxstep;
     A.tend = p.tend;
     A.rint = p.rint;

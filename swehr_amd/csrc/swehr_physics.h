@@ -28,9 +28,33 @@ __device__ __forceinline__ double msqrt(double x)
     g = __builtin_fma(d, h, g);
     return x == 0.0 ? x : g;
 }
-__device__ __forceinline__ float msqrt(float x) { return __builtin_sqrtf(x); }
-__device__ __forceinline__ double mrsqrt(double x) { return rsqrt(x); }
-__device__ __forceinline__ float mrsqrt(float x) { return rsqrtf(x); }
+// FP32: the hardware square root and reciprocal square root (1 ulp), without the
+// library's correction to the correctly rounded result (half of the error budget of a
+// form that stores FP32 anyway, at ~15 instructions each).
+__device__ __forceinline__ float msqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+// The same for x > 0 (settling discriminants), without the screening of x = 0.
+__device__ __forceinline__ double msqrt_pos(double x)
+{
+    const double r = __builtin_amdgcn_rsq(x);
+    double g = x * r, h = 0.5 * r;
+    const double e = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, e, g);
+    h = __builtin_fma(h, e, h);
+    const double d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+}
+__device__ __forceinline__ float msqrt_pos(float x) { return __builtin_amdgcn_sqrtf(x); }
+// 1/sqrt(x) for x >= 1 (every use: 1 + squared slopes): the hardware estimate y
+// refined to y (1 + t/2 + 3t^2/8) with t = 1 - x y^2, the library's own sequence
+// minus its screening of zero and infinite x, which such x never need (the results
+// are those of rsqrt).
+__device__ __forceinline__ double mrsqrt(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    const double t = __builtin_fma(y * -x, y, 1.0);
+    return __builtin_fma(y * t, __builtin_fma(t, 0.375, 0.5), y);
+}
+__device__ __forceinline__ float mrsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ double mfabs(double x) { return __builtin_fabs(x); }
 __device__ __forceinline__ float mfabs(float x) { return __builtin_fabsf(x); }
 __device__ __forceinline__ double mfmin(double a, double b) { return __builtin_fmin(a, b); }
