@@ -406,6 +406,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     double *s_z = reinterpret_cast<double *>(smem_raw), *s_zp = s_z + ZEXT;
     R *s_h = reinterpret_cast<R *>(s_zp + ZEXT);
     __shared__ double s_exp2[32];  // 2^(j/32) for the FP64 exponentials of phase 3
+    __shared__ double s_log[69];   // kLogTab for the FP64 logarithms of phase 3
     R *s_u = s_h + EXT, *s_v = s_u + EXT, *s_ts = s_v + EXT, *s_dep = s_ts + EXT;
     R *s_fx = s_dep + EXT, *s_fy = s_fx + 3 * NFX;
     R *s_sed = s_fy + 3 * NFY;
@@ -415,6 +416,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     const R inv_rhos = R(1.0 / kRhos);
     const int tid = threadIdx.x;
     if (sizeof(R) == sizeof(double) && tid < 32) s_exp2[tid] = kExp2Frac[tid];
+    if (sizeof(R) == sizeof(double) && tid < 69) s_log[tid] = (&kLogTab[0][0])[tid];
 
     // Own-cell values of the update, loaded first so that their latency overlaps the
     // halo staging below (every load of a phase is issued before the first wait).
@@ -747,7 +749,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         if (active) {
             const R qq = hu * hu + hv * hv;
             const R qm = msqrt(qq);
-            const R sf = manning2_over_hpow<3>(h, rough, s_exp2) * qq;
+            const R sf = manning2_over_hpow<3>(h, rough, s_exp2, s_log) * qq;
             omega = rho * gz * sf * qm;
             const R oneH = R(1) - H;
             f1 = oneH * R(A.a) * Rr * shield;
@@ -981,7 +983,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         // implicit friction (SWEHR.c:1553-1565) and maxima (SWEHR.c:1587-1588)
         R v1 = R(0), v2 = R(0);
         if (hn > hfilm) {
-            const R sf2 = manning2_over_hpow<2>(hn, rough, s_exp2) * gz * msqrt(hun * hun + hvn * hvn);
+            const R sf2 = manning2_over_hpow<2>(hn, rough, s_exp2, s_log) * gz * msqrt(hun * hun + hvn * hvn);
             const R den = mrcp(R(1) + dt * sf2);
             hun = hun * den;
             hvn = hvn * den;

@@ -97,6 +97,65 @@ __device__ __forceinline__ double exp_tab(double x, const double *tab)
     return __builtin_ldexp(tab[j] * p, m);
 }
 __device__ __forceinline__ float mexp(float x) { return expf(x); }
+
+// For F = 1 + j/32, j = -9..13: 1/F rounded, and ln F as a rounded head plus the
+// rounded rest (the table of log_tab, staged in LDS).
+__constant__ double kLogTab[23][3] = {
+    {0x1.642c8590b2164p+0, -0x1.522ae0738a3d8p-2, 0x1.8f7e9b38a6979p-57},
+    {0x1.5555555555555p+0, -0x1.269621134db92p-2, -0x1.e0efadd9db02bp-56},
+    {0x1.47ae147ae147bp+0, -0x1.f991c6cb3b379p-3, -0x1.f665066f980a2p-57},
+    {0x1.3b13b13b13b14p+0, -0x1.a93ed3c8ad9e3p-3, -0x1.bcafa9de97203p-57},
+    {0x1.2f684bda12f68p+0, -0x1.5bf406b543db2p-3, 0x1.1f5b44c0df7e7p-61},
+    {0x1.2492492492492p+0, -0x1.1178e8227e47cp-3, 0x1.0e63a5f01c691p-58},
+    {0x1.1a7b9611a7b96p+0, -0x1.9335e5d594989p-4, 0x1.478a85704ccb7p-58},
+    {0x1.1111111111111p+0, -0x1.08598b59e3a07p-4, 0x1.dd7009902bf32p-58},
+    {0x1.0842108421084p+0, -0x1.0415d89e74444p-5, -0x1.c05cf1d753622p-59},
+    {0x1.0000000000000p+0, 0x0.0p+0, 0x0.0p+0},
+    {0x1.f07c1f07c1f08p-1, 0x1.f829b0e783300p-6, 0x1.33e3f04f1ef23p-60},
+    {0x1.e1e1e1e1e1e1ep-1, 0x1.f0a30c01162a6p-5, 0x1.85f325c5bbacdp-59},
+    {0x1.d41d41d41d41dp-1, 0x1.6f0d28ae56b4cp-4, -0x1.906d99184b992p-58},
+    {0x1.c71c71c71c71cp-1, 0x1.e27076e2af2e6p-4, -0x1.61578001e0162p-60},
+    {0x1.bacf914c1bad0p-1, 0x1.29552f81ff523p-3, 0x1.301771c407dbfp-57},
+    {0x1.af286bca1af28p-1, 0x1.5ff3070a793d4p-3, -0x1.bc60efafc6f6ep-58},
+    {0x1.a41a41a41a41ap-1, 0x1.9525a9cf456b4p-3, 0x1.d904c1d4e2e26p-57},
+    {0x1.999999999999ap-1, 0x1.c8ff7c79a9a22p-3, -0x1.4f689f8434012p-57},
+    {0x1.8f9c18f9c18fap-1, 0x1.fb9186d5e3e2bp-3, -0x1.caaae64f21acbp-57},
+    {0x1.8618618618618p-1, 0x1.1675cababa60ep-2, 0x1.ce63eab883717p-61},
+    {0x1.7d05f417d05f4p-1, 0x1.2e8e2bae11d31p-2, -0x1.8f4cdb95ebdf9p-56},
+    {0x1.745d1745d1746p-1, 0x1.4618bc21c5ec2p-2, 0x1.f42decdeccf1dp-56},
+    {0x1.6c16c16c16c17p-1, 0x1.5d1bdbf5809cap-2, 0x1.4236383dc7fe1p-56},
+};
+constexpr double kLn2Hi = 0x1.62e42fefa3800p-1;  // ln 2 to 42 bits: m kLn2Hi is exact
+constexpr double kLn2Lo = 0x1.ef35793c76730p-45;
+
+// FP64 natural log of a positive normal x from the table tab (kLogTab in LDS): x = 2^m f
+// with f in [sqrt(1/2), sqrt(2)), F = 1 + j/32 the nearest table point (f - F is exact),
+// r = (f - F)/F by the tabulated 1/F, ln(1 + r) by its Taylor polynomial to r^9
+// (|r| <= 1/64: truncation < 2^-54 relative), and ln x = m ln2 + ln F + ln(1 + r) summed
+// small terms first.  Within 1 ulp of the C library's log (checked on 2e7 arguments
+// over 1e-7..1e3) at well under half the instructions of the library log.
+__device__ __forceinline__ double log_tab(double x, const double *tab)
+{
+    double f = 2.0 * __builtin_amdgcn_frexp_mant(x);
+    int m = __builtin_amdgcn_frexp_exp(x) - 1;
+    if (f >= 0x1.6a09e667f3bcdp+0) {
+        f *= 0.5;
+        m += 1;
+    }
+    const double jd = __builtin_rint((f - 1.0) * 32.0);
+    const int j = (int)jd + 9;
+    const double r = (f - __builtin_fma(jd, 1.0 / 32, 1.0)) * tab[3 * j];
+    double q = __builtin_fma(r, 1.0 / 9, -1.0 / 8);
+    q = __builtin_fma(r, q, 1.0 / 7);
+    q = __builtin_fma(r, q, -1.0 / 6);
+    q = __builtin_fma(r, q, 1.0 / 5);
+    q = __builtin_fma(r, q, -1.0 / 4);
+    q = __builtin_fma(r, q, 1.0 / 3);
+    q = __builtin_fma(r, q, -0.5);
+    const double p = __builtin_fma(r * r, q, r);
+    const double dm = (double)m;
+    return __builtin_fma(dm, kLn2Hi, tab[3 * j + 1] + (p + __builtin_fma(dm, kLn2Lo, tab[3 * j + 2])));
+}
 // 1/x for x finite and nonzero: the hardware estimate refined by two Newton steps in
 // FP64 (within 1 ulp, half the instructions of IEEE division); the 1-ulp hardware
 // reciprocal in FP32.  Where x = 0 is possible the result is NaN (not inf): callers use
@@ -183,11 +242,11 @@ constexpr double kLog2HcFriction = -8.380821783940931;
 // so its rounding costs ~1e-7 relative instead of the ~3e-6 of exp(-3.333 ln h) at
 // h = 1e-3; the hardware log2/exp2 (1 ulp) and reciprocal are then accurate enough.
 template <int PI, typename R>
-__device__ __forceinline__ R manning2_over_hpow(R h, R n0, const double *exp_table)
+__device__ __forceinline__ R manning2_over_hpow(R h, R n0, const double *exp_table, const double *log_table)
 {
     if constexpr (sizeof(R) == sizeof(double)) {
         const double p = PI == 3 ? 3.333 : 2.333;
-        const double lh = log(h);
+        const double lh = log_tab(h, log_table);
         double ex = -p * lh;
         if (h <= kHcFriction) ex += 2 * kDepthExp * (lh - kLnHcFriction);
         return n0 * n0 * exp_tab(ex, exp_table);
