@@ -340,6 +340,11 @@ __device__ __forceinline__ void wave_speed_tile(const StepArgs<R> &A, int tile)
 template <typename R, bool SPLIT = false>
 constexpr int kStepWaves = sizeof(R) == 4 ? 5 : 3;
 
+// FP32: classes whose M_k the step kernel loads with the own-cell values (the rest
+// one class ahead in the class loop); the 96-VGPR budget of 5 waves/SIMD spills from
+// 2 on.
+constexpr int kFp32MOwn = 1;
+
 // Size classes whose upwind concentrations have LDS planes of their own (the first
 // chunk); in FP64 the classes above share the planes of the velocities, TOTSED, DEP
 // and face fluxes, which are dead by the time the class loop reaches them.  The FP64
@@ -451,7 +456,15 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     md_o = ld_stream(maxdepthp, cRL);
     vinf_o = ld_stream(vinfp, c8L);
     if (A.ks_raster) ks_o = ld_stream(gmem(A.ks_raster), c8L);
-    R m_a = ld_stream(Mp, cRL);
+    // M_k of the own cell for the first MO classes, loaded here with the other own-cell
+    // values so that their latency hides behind phases 1-2 (one class ahead inside the
+    // class loop it did not: 0.428 -> 0.403 ms on the 2048^2, S = 7 FP64 step); the
+    // classes above MO load M_k one class ahead
+    constexpr int MO = sizeof(R) == sizeof(double) ? KO : kFp32MOwn;
+    R m_own[MO > 0 ? MO : 1];
+#pragma unroll
+    for (int k = 0; k < MO; k++) m_own[k] = k < S ? ld_stream(Mp + k * plane, cRL) : R(0);
+    R m_a = MO < S ? ld_stream(Mp + MO * plane, cRL) : R(0);
 
     // ---- phase 1: ghosts and the 1-cell halo of the conserved state (SWEHR.c:816-1035)
     constexpr int EIT = (EXT + NT - 1) / NT, ZIT = (ZEXT + NT - 1) / NT;
@@ -878,21 +891,23 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
                 st_stream(Mp + k * plane, cR, m);
             }
         };
-        // first chunk, classes 0 .. S0-1.  Below KO: unrolled, ch_k from registers,
-        // M_{k+1} loaded before class k; from KO on ch_k and M_k come from memory, one
-        // class ahead.
+        // first chunk, classes 0 .. S0-1.  Below KO: unrolled, ch_k from registers; from
+        // KO on ch_k comes from memory, one class ahead.  M_k from registers below MO.
         const int S0 = S < KS ? S : KS;
 #pragma unroll
-        for (int k = 0; k < KO; k++) {
-            if (k < S0) {
-                const R m_k = m_a;
-                if (k + 1 < S) m_a = ld_stream(Mp + (k + 1) * plane, cRL);
-                update_class(k, ch_own[k], m_k, s_sed + k * EXT);
-            }
-        }
+        for (int k = 0; k < KO; k++)
+            if (k < S0) update_class(k, ch_own[k], m_own[k], s_sed + k * EXT);
         const int kf = KO < S0 ? KO : KS;  // first class whose ch_k is read below
         R ch_a = kf < S ? at(chin + kf * plane, cRL) : R(0);
-        for (int k = KO; k < S0; k++) {
+#pragma unroll
+        for (int k = KO; k < MO; k++) {
+            if (k < S0) {
+                const R ch_k = ch_a;
+                if (k + 1 < S) ch_a = at(chin + (k + 1) * plane, cRL);
+                update_class(k, ch_k, m_own[k], s_sed + k * EXT);
+            }
+        }
+        for (int k = (KO > MO ? KO : MO); k < S0; k++) {
             const R ch_k = ch_a, m_k = m_a;
             if (k + 1 < S) {
                 ch_a = at(chin + (k + 1) * plane, cRL);
