@@ -344,6 +344,9 @@ constexpr int kStepWaves = sizeof(R) == 4 ? 5 : 3;
 // one class ahead in the class loop); the 96-VGPR budget of 5 waves/SIMD spills from
 // 2 on.
 constexpr int kFp32MOwn = 1;
+// FP64 two-chunk variant: classes whose M_k is loaded up front (to 10 classes within
+// its register budget; the first chunk's 7 and three of the second)
+constexpr int kSplitMOwn = 10;
 
 // Size classes whose upwind concentrations have LDS planes of their own (the first
 // chunk); in FP64 the classes above share the planes of the velocities, TOTSED, DEP
@@ -460,7 +463,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     // values so that their latency hides behind phases 1-2 (one class ahead inside the
     // class loop it did not: 0.428 -> 0.403 ms on the 2048^2, S = 7 FP64 step); the
     // classes above MO load M_k one class ahead
-    constexpr int MO = sizeof(R) == sizeof(double) ? KO : kFp32MOwn;
+    constexpr int MO = sizeof(R) == sizeof(double) ? (SPLIT ? kSplitMOwn : KO) : kFp32MOwn;
     R m_own[MO > 0 ? MO : 1];
 #pragma unroll
     for (int k = 0; k < MO; k++) m_own[k] = k < S ? ld_stream(Mp + k * plane, cRL) : R(0);
@@ -966,7 +969,15 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
                 }
             }
             __syncthreads();
-            for (int k = KS; k < S; k++) {
+#pragma unroll
+            for (int k = KS; k < MO; k++) {
+                if (k < S) {
+                    const R ch_k = ch_a;
+                    if (k + 1 < S) ch_a = at(chin + (k + 1) * plane, cRL);
+                    update_class(k, ch_k, m_own[k], s_sed1 + (k - KS) * EXT);
+                }
+            }
+            for (int k = (MO > KS ? MO : KS); k < S; k++) {
                 const R ch_k = ch_a, m_k = m_a;
                 if (k + 1 < S) {
                     ch_a = at(chin + (k + 1) * plane, cRL);
