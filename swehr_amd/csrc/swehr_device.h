@@ -443,7 +443,6 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     R hu_o, hv_o, totm_o, mv_o, md_o;
     double vinf_o, ks_o = A.ks;
     const int lane = tid & 63, wid = tid >> 6;
-    // ch_k and M_k of the first class (the class loop prefetches one class ahead)
     // ch_k of the own cell for the first KO classes, loaded with the halo staging (the
     // same lines, so the requests merge in L2) and kept in registers for phase 3: read
     // again there they would come from memory, the halo having left L2 by then
