@@ -239,6 +239,10 @@ int swehr_batch_advance(swehr_batch *b, int64_t max_steps, int *remaining);
 int swehr_batch_handle(swehr_batch *b, int i, swehr_handle **h);
 /* Autotune the shared tile shape over the whole ensemble (as swehr_autotune). */
 int swehr_batch_autotune(swehr_batch *b, int nsteps, int *best, double *ms_per_tile);
+/* Diagnostic, no counterpart in the reference: nsteps ensemble steps launched one by
+ * one with HIP events around each launch; avg_ms[0..2] receives the average duration
+ * of the wave-speed, step and control launches (the steps advance the ensemble). */
+int swehr_batch_profile(swehr_batch *b, int nsteps, double *avg_ms);
 int swehr_batch_destroy(swehr_batch *b);
 
 /* ---- serving: continuous batching of solve requests.  Requests wait in a queue

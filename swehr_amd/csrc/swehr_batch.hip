@@ -177,6 +177,43 @@ struct Engine {
         return SWEHR_OK;
     }
 
+    // Eager launches of one ensemble step each, with events between them; adds the
+    // three launch durations (ms) to sum[0..2].
+    int profile_steps(int steps, double *sum)
+    {
+        const int n = (int)members.size();
+        if (!n) return SWEHR_OK;
+        hipEvent_t ev[4];
+        for (int k = 0; k < 4; k++) HIPCHK(hipEventCreate(&ev[k]));
+        const int blocks = 8 * per_xcd;
+        int rc = SWEHR_OK;
+        for (int s = 0; s < steps && rc == SWEHR_OK; s++) {
+            (void)hipEventRecord(ev[0], stream);
+            launch_batch_wave(tile, precision, blocks, stream, d_args, d_map, n_map, per_xcd);
+            (void)hipEventRecord(ev[1], stream);
+            launch_batch_step(tile, precision, blocks, lds, stream, d_args, d_map, n_map, per_xcd, max_classes);
+            (void)hipEventRecord(ev[2], stream);
+            if (precision == SWEHR_FP64)
+                hipLaunchKernelGGL((batch_control_kernel<double>), dim3(n), dim3(256), 0, stream,
+                                   (const StepArgs<double> *)d_args, (const int *)d_insts, n);
+            else
+                hipLaunchKernelGGL((batch_control_kernel<float>), dim3(n), dim3(256), 0, stream,
+                                   (const StepArgs<float> *)d_args, (const int *)d_insts, n);
+            (void)hipEventRecord(ev[3], stream);
+            if (hipEventSynchronize(ev[3]) != hipSuccess) {
+                rc = SWEHR_ERR_HIP;
+                break;
+            }
+            for (int k = 0; k < 3; k++) {
+                float ms = 0;
+                (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+                sum[k] += ms;
+            }
+        }
+        for (int k = 0; k < 4; k++) (void)hipEventDestroy(ev[k]);
+        return rc;
+    }
+
     int enqueue_steps(int steps)
     {
         const int n = (int)members.size();
@@ -440,6 +477,21 @@ extern "C" int swehr_batch_autotune(swehr_batch *b, int nsteps, int *best, doubl
     E.dirty = true;
     if (best) *best = use;
     return rc;
+}
+
+extern "C" int swehr_batch_profile(swehr_batch *b, int nsteps, double *avg_ms)
+{
+    if (!b || nsteps < 1 || !avg_ms) return SWEHR_ERR_BAD_INPUT;
+    Engine &E = b->eng;
+    avg_ms[0] = avg_ms[1] = avg_ms[2] = 0;
+    if (E.members.empty()) return SWEHR_OK;
+    int rc = SWEHR_OK;
+    if (E.dirty && (rc = E.rebuild()) != SWEHR_OK) return rc;
+    if ((rc = E.set_limits(-1)) != SWEHR_OK) return rc;
+    double sum[3] = {0, 0, 0};
+    if ((rc = E.profile_steps(nsteps, sum)) != SWEHR_OK) return rc;
+    for (int k = 0; k < 3; k++) avg_ms[k] = sum[k] / nsteps;
+    return SWEHR_OK;
 }
 
 extern "C" int swehr_batch_destroy(swehr_batch *b)

@@ -81,6 +81,7 @@ def lib():
     L.swehr_batch_advance.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_int)]
     L.swehr_batch_handle.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
     L.swehr_batch_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), P]
+    L.swehr_batch_profile.argtypes = [C.c_void_p, C.c_int, P]
     L.swehr_batch_destroy.argtypes = [C.c_void_p]
     L.swehr_server_create.argtypes = [C.c_int64, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
     L.swehr_server_submit.argtypes = [C.c_void_p, C.POINTER(_abi.Params), P, P, P, P, P, C.POINTER(C.c_int64)]
@@ -341,6 +342,13 @@ class Batch:
         ms = np.zeros(lib().swehr_tile_count())
         check("swehr_batch_autotune", lib().swehr_batch_autotune(self._b, int(nsteps), C.byref(best), dptr(ms)))
         return best.value, ms
+
+    def profile(self, nsteps: int = 10):
+        """Average durations (ms) of the ensemble's wave-speed, step and control launches
+        over nsteps eagerly launched steps (which advance the ensemble)."""
+        out = np.zeros(3)
+        check("swehr_batch_profile", lib().swehr_batch_profile(self._b, int(nsteps), dptr(out)))
+        return out
 
     def close(self):
         if getattr(self, "_b", None):

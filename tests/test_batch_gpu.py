@@ -88,6 +88,19 @@ def test_ensemble_mixed_class_counts_equal_alone_runs():
                 _same(b.member(i), s, c)
 
 
+def test_batch_profile_times_each_launch_and_advances():
+    """The ensemble diagnostic: positive durations of the three launches, and every
+    unfinished member advanced by exactly the profiled steps."""
+    cases = [_golden("transmissive"), _golden("closed_basin")]
+    with sv.Batch(cases, precision=_abi.FP32) as b:
+        b.advance(10)
+        before = [b.member(i).info()["steps"] for i in range(len(cases))]
+        ms = b.profile(5)
+        assert ms.shape == (3,) and all(x > 0 for x in ms)
+        after = [b.member(i).info()["steps"] for i in range(len(cases))]
+        assert after == [x + 5 for x in before]
+
+
 def test_ensemble_fp64_matches_oracle():
     cases = [_golden("transmissive"), _golden("closed_basin")]
     with sv.Batch(cases, precision=_abi.FP64) as b:
