@@ -420,7 +420,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1200)
     ap.add_argument("--warmup", type=int, default=2400)
-    ap.add_argument("--grid", type=int, default=2048)
+    ap.add_argument("--grid", type=int, default=0, help="grid side (default 2048, or 4096 with --case c3)")
+    ap.add_argument("--case", choices=["c2", "c3"], default="c2",
+                    help="headline workload: the 2048^2, S=7 fractal plane (c2, the metric's configuration) or "
+                         "the 4096^2, S=10 channel network of the precision form (c3)")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -462,12 +465,14 @@ def main():
 
     from swehr_amd import _abi
     from swehr_amd import solver as sv
-    from swehr_amd.inputs import config_c2
+    from swehr_amd.inputs import config_c2, config_c3
 
     prec = _abi.FP64 if args.precision == "fp64" else _abi.FP32
     w = 8 if prec == _abi.FP64 else 4
     t0 = time.perf_counter()
-    case = config_c2(n=args.grid)
+    if not args.grid:
+        args.grid = 4096 if args.case == "c3" else 2048
+    case = config_c2(n=args.grid) if args.case == "c2" else config_c3(n=args.grid)
     log("case %s: %d x %d, S=%d built in %.1f s" % (case.name, case.nx, case.ny, case.nclass, time.perf_counter() - t0))
     hip = Hip()
     S = case.nclass
@@ -492,7 +497,7 @@ def main():
     bcell = bytes_per_cell(S, w)
     step_ms = float(kms[1])
     achieved = cells * bcell / (step_ms * 1e-3) / 1e9
-    key = "c2_%d_%s" % (args.grid, args.precision)
+    key = "%s_%d_%s" % (args.case, args.grid, args.precision)
     traffic = traffic_for(key)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -544,8 +549,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if prec == _abi.FP64 else "f32",
         "data": "synthetic (seeded diamond-square topography, log-normal Ks raster)",
-        "config": {"workload": "plain %s step, %dx%d cells, dx 1 m, S=%d, 80 mm/h rain, per-cell Ks"
-                               % (args.precision.upper(), args.grid, args.grid, S),
+        "config": {"workload": ("plain %s step, %dx%d cells, dx 1 m, S=%d, 80 mm/h rain, per-cell Ks"
+                                if args.case == "c2" else
+                                "plain %s step, %dx%d cells, dx 1 m, channel network, S=%d, triangular hyetograph, "
+                                "per-cell Ks") % (args.precision.upper(), args.grid, args.grid, S),
                    "grid": [case.nx, case.ny], "classes": S, "precision": args.precision,
                    "t_end_s": info["t"], "tile": args.tile or "autotuned",
                    "autotune_step_kernel_ms": None if tile_ms is None else [round(float(x), 4) for x in tile_ms]},

@@ -816,6 +816,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         const R hdt = R(0.5) * dt;
         double zn_c = s_z[zi * ZJ + zj];
         R totm = R(0), depn = R(0), chtemp = R(0);
+        bool left_suspended = false;  // some class ends the step with ch_k != 0
         double sb_out = 0, sq_out = 0;  // sediment leaving through all edges / through j = 1/2
         const bool edge = gi == 0 || gi == nx - 1 || gj == 0 || gj == ny - 1;
         // class k from its suspended mass ch_k and deposited mass m_k
@@ -888,6 +889,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             bud[B_EXCH] += (double)dh;
             totm += m;
             depn += m * inv_rhos;
+            left_suspended = left_suspended || u != R(0);
             if (own) {
                 st_stream(chout + k * plane, cR, u);
                 st_stream(Mp + k * plane, cR, m);
@@ -992,9 +994,13 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             hn = hfilm;
             hun = R(0);
             hvn = R(0);
+            // the suspended mass goes to the deposited layer and the bed; where every
+            // class ended with ch_k = 0 (a cell dry through the whole step, the common
+            // case) that changes nothing but the sign of a zero, and the classes are not
+            // read back and rewritten, which would double the cell's traffic
             glob<R> *chq = chout;
-            R u_a = at(chq, cR), mm_a = at(Mp, cR);
-            for (int k = 0; k < S; k++) {
+            R u_a = left_suspended ? at(chq, cR) : R(0), mm_a = left_suspended ? at(Mp, cR) : R(0);
+            for (int k = 0; k < (left_suspended ? S : 0); k++) {
                 const R u = u_a, mm = mm_a;
                 if (k + 1 < S) {
                     u_a = at(chq + (k + 1) * plane, cR);
