@@ -134,6 +134,47 @@ def cpu_baseline(case, budget_s: float, threads: int):
     return 1 + more, el
 
 
+def reference_binary_baseline(case, budget_s: float, step_s_estimate: float):
+    """The compiled reference on the same case: SWEHR.c built from its source by
+    oracle/Makefile with the parameter hooks of oracle/ref_hooks.h (inert unless set;
+    here they carry the case's Ks raster, soil values, courant number and switches).
+    It is single-threaded: its OpenMP pragmas are commented out (SWEHR.c:1091, 1416).
+    One run stops after 1 step and one after 1 + n steps; the difference of their wall
+    times is n steps, so parsing, set-up and the final raster writes cancel.  Returns
+    (n, seconds, seconds of the 1-step run), or None where the build is absent."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    binary = os.path.join(ROOT, "oracle", "_ref", "swehr_ref_hooked")
+    if not os.path.exists(binary):
+        return None
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    from cases import hook_env
+    from swehr_amd.inputs import write_case
+
+    d = tempfile.mkdtemp(prefix="swehr_refbase_")
+    try:
+        write_case(case, d)
+
+        def run(maxsteps):
+            env = dict(os.environ)
+            env.update(hook_env(case, d, maxsteps))
+            t0 = time.perf_counter()
+            p = subprocess.run([binary], cwd=d, env=env, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE,
+                               timeout=900)
+            if p.returncode != 0:
+                raise RuntimeError("reference build failed (%d): %s" % (p.returncode, p.stderr[-500:]))
+            return time.perf_counter() - t0
+
+        n = int(max(2, min(200, budget_s / max(step_s_estimate, 1e-6))))
+        t_one = run(1)
+        t_more = run(1 + n)
+        return n, t_more - t_one, t_one
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 def parity_c1(threads: int):
     """The reference's CPU-runnable configuration (100 x 40 plane, 1200 s) run in full by
     the FP64 oracle on the host cores (timed: the CPU baseline at this size) and by the
@@ -428,6 +469,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ref-binary", action="store_true",
+                    help="skip timing the compiled reference (oracle/_ref) beside the OpenMP port")
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--steps-per-graph", type=int, default=16)
     ap.add_argument("--tile", type=int, default=0)
@@ -519,6 +562,16 @@ def main():
                "one_thread": {"value": cells * S * n1 / el1, "steps": n1, "seconds": round(el1, 2)},
                "host": host_description(),
                "parity_c1": parity_c1(threads)}
+        if not args.no_ref_binary:
+            rb = reference_binary_baseline(case, args.cpu_seconds / 2, el1 / n1)
+            if rb is not None:
+                nr, elr, t_one = rb
+                cpu["reference_binary"] = {
+                    "value": cells * S * nr / elr, "unit": "cell-class-steps/s", "cores": 1, "kind": "reference",
+                    "sample": "SWEHR.c compiled from source (oracle/_ref/swehr_ref_hooked, gcc -O2), %d steps of the "
+                              "same case from t=0: wall time of a %d-step run minus a 1-step run (%.1f s, which "
+                              "holds parsing, set-up and the final raster writes), %.1f s" % (nr, nr + 1, t_one, elr),
+                    "port_one_thread_over_reference": round(cells * S * n1 / el1 / (cells * S * nr / elr), 3)}
         log("cpu baseline: %s" % cpu)
 
     forms = {}
