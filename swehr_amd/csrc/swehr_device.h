@@ -343,7 +343,7 @@ __device__ __forceinline__ void wave_speed_tile(const StepArgs<R> &A, int tile)
 // Waves per SIMD the step kernels are compiled for (VGPR budget 512 / waves): the
 // LDS of a tile admits 3 blocks per CU in FP64 and 5 in FP32 at the default tile.
 template <typename R, bool SPLIT = false>
-constexpr int kStepWaves = sizeof(R) == 4 ? 5 : 3;
+constexpr int kStepWaves = sizeof(R) == 4 ? (SPLIT ? 4 : 5) : 3;
 
 // FP32: classes whose M_k the step kernel loads with the own-cell values (the rest
 // one class ahead in the class loop); the 96-VGPR budget of 5 waves/SIMD spills from
@@ -352,6 +352,9 @@ constexpr int kFp32MOwn = 1;
 // FP64 two-chunk variant: classes whose M_k is loaded up front (to 10 classes within
 // its register budget; the first chunk's 7 and three of the second)
 constexpr int kSplitMOwn = 10;
+// FP32 variant for many classes: classes whose ch_k and M_k of the own cell stay in
+// registers (within the 128-VGPR budget of 4 waves per SIMD)
+constexpr int kFp32WideOwn = 10;
 
 // Size classes whose upwind concentrations have LDS planes of their own (the first
 // chunk); in FP64 the classes above share the planes of the velocities, TOTSED, DEP
@@ -385,7 +388,12 @@ struct TileGeom {
 // reciprocal computed once and multiplied, 1/sqrt via rsqrt, c* = |d|/4 in the
 // Riemann solver, the class-invariant source factors hoisted out of the class loop);
 // those change results at the rounding level only.
-// SPLIT: the variant for any S up to kMaxClasses; without it S <= kLdsClasses.
+// SPLIT: the variant for many classes.  In FP64 it serves any S up to kMaxClasses by
+// staging the classes beyond kLdsClasses in a second LDS chunk (without it S <=
+// kLdsClasses).  In FP32, where a tile's LDS for that many classes admits 4 blocks
+// per CU instead of 5, it is compiled for 4 waves per SIMD and spends the registers
+// this frees on the own cell's ch_k and M_k of the first classes, which are then read
+// once (with the halo staging) instead of again in the class loop.
 template <typename R, int TI, int TJ, bool SPLIT>
 __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsigned char *smem_raw)
 {
@@ -451,8 +459,9 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     // ch_k of the own cell for the first KO classes, loaded with the halo staging (the
     // same lines, so the requests merge in L2) and kept in registers for phase 3: read
     // again there they would come from memory, the halo having left L2 by then
-    // (FP64 only: the FP32 kernel's register budget for 5 waves per SIMD has no room)
-    constexpr int KO = sizeof(R) == sizeof(double) ? KS : 0;
+    // (in FP32 only in the SPLIT variant: the register budget of 5 waves per SIMD has
+    // no room)
+    constexpr int KO = sizeof(R) == sizeof(double) ? KS : (SPLIT ? kFp32WideOwn : 0);
     R ch_own[KO > 0 ? KO : 1];
 #pragma unroll
     for (int k = 0; k < KO; k++) ch_own[k] = k < S ? at(chin + k * plane, cRL) : R(0);
@@ -467,7 +476,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     // values so that their latency hides behind phases 1-2 (one class ahead inside the
     // class loop it did not: 0.428 -> 0.403 ms on the 2048^2, S = 7 FP64 step); the
     // classes above MO load M_k one class ahead
-    constexpr int MO = sizeof(R) == sizeof(double) ? (SPLIT ? kSplitMOwn : KO) : kFp32MOwn;
+    constexpr int MO = sizeof(R) == sizeof(double) ? (SPLIT ? kSplitMOwn : KO) : (SPLIT ? kFp32WideOwn : kFp32MOwn);
     R m_own[MO > 0 ? MO : 1];
 #pragma unroll
     for (int k = 0; k < MO; k++) m_own[k] = k < S ? ld_stream(Mp + k * plane, cRL) : R(0);
@@ -677,7 +686,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     // classes every thread of the block takes part (the second chunk of classes is
     // staged between two barriers); threads outside the domain store nothing.
     double bud[NB_TERMS] = {0, 0, 0, 0, 0};
-    if (own || (SPLIT && S > KS)) {
+    constexpr bool TWO_CHUNK = SPLIT && KS < kMaxClasses;
+    if (own || (TWO_CHUNK && S > KS)) {
         const int e = (il + 1) * EJ + (jl + 1);
         const int zi = il + 2, zj = jl + 2;
         const R h = s_h[e], hu = hu_o, hv = hv_o;
@@ -824,6 +834,21 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         R totm = R(0), depn = R(0), chtemp = R(0);
         bool left_suspended = false;  // some class ends the step with ch_k != 0
         double sb_out = 0, sq_out = 0;  // sediment leaving through all edges / through j = 1/2
+        // FP32: the exchange with the bed changes h by ~1e-10 m per class and step, far
+        // below the spacing of floats at h (9.3e-10 m at 1 cm) and mostly of one sign, so
+        // added to h one by one it would be lost every step (a systematic water budget
+        // error).  The classes' increments are also summed apart, and their sum is added
+        // to the depth from before the class loop once, with one unbiased rounding.
+        const R hn_pre = hn;
+        R dh_sum = R(0);
+        // random bits of the unbiased roundings (FP32): a hash of the cell and the step,
+        // then a linear congruential sequence over the values the cell stores, so results
+        // are reproducible and do not depend on the tile shape or on the ensemble
+        unsigned rbits = sizeof(R) == sizeof(float) ? mix32(c ^ mix32((unsigned)steps)) : 0u;
+        auto next_bits = [&]() {
+            rbits = rbits * 1664525u + 1013904223u;
+            return rbits;
+        };
         const bool edge = gi == 0 || gi == nx - 1 || gj == 0 || gj == ny - 1;
         // class k from its suspended mass ch_k and deposited mass m_k
         auto update_class = [&](int k, R ch_k, R m_k, const R *sk) {
@@ -866,21 +891,39 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             if (((A.coarse2 >> k) & 1u) && shallow_v) vf = R(2e-3) + vshal * (vf - R(2e-3));
             vf = vf * vf_keep;
             // exchange with the deposited layer
-            if (hn < R(10) * hfilm) {
-                m += dt * e1 + u;
+            // the running depth: in FP32 the depth before the loop plus the increments so far
+            const R hcur = sizeof(R) == sizeof(double) ? hn : hn_pre + dh_sum;
+            if (hcur < R(10) * hfilm) {
+                if constexpr (sizeof(R) == sizeof(double)) m += dt * e1 + u;
+                else m = add_unbiased(m, dt * e1 + u, next_bits());
                 bud[B_DET] += (double)(dt * e1);
                 u = R(0);
             } else {
                 chtemp = u;
                 const R red = mfmin(m, dt * (e2 + e4));
-                R decay;
-                if constexpr (sizeof(R) == sizeof(double)) decay = exp_tab(-vf * hdt * mrcp(hn), s_exp2);
-                else decay = mexp(-vf * hdt * mrcp(hn));
-                u = decay * u;
-                u = u + dt * (e1 + e3) + red;
-                u = decay * u;
-                zn_c = zn_c - (double)(kbed * (u - chtemp));
-                m = m + dt * (e1 + e3) - (u - chtemp);
+                if constexpr (sizeof(R) == sizeof(double)) {
+                    const R decay = exp_tab(-vf * hdt * mrcp(hcur), s_exp2);
+                    u = decay * u;
+                    u = u + dt * (e1 + e3) + red;
+                    u = decay * u;
+                    zn_c = zn_c - (double)(kbed * (u - chtemp));
+                    m = m + dt * (e1 + e3) - (u - chtemp);
+                } else {
+                    // FP32: the decay factor d = exp(-vf dt / 2h) of a fine class differs
+                    // from 1 by less than the spacing of floats there (2 um at h = 1 cm:
+                    // 1 - 4.5e-8), so d * u as written rounds to u or overstates the
+                    // settling.  The update d (d u + s) is taken as one increment of u,
+                    // u em1 (2 + em1) + s (1 + em1) with em1 = d - 1 (mexpm1_neg), added
+                    // with one unbiased rounding; M_k takes the difference of the stored
+                    // values and is updated the same way
+                    const R em1 = mexpm1_neg(-vf * hdt * mrcp(hcur));
+                    const R src = dt * (e1 + e3) + red;
+                    const R inc = u * (em1 * (R(2) + em1)) + src * (R(1) + em1);
+                    u = add_unbiased(u, inc, next_bits());
+                    const R dch = u - chtemp;
+                    zn_c = zn_c - (double)(kbed * dch);
+                    m = add_unbiased(m, dt * (e1 + e3) - dch, next_bits());
+                }
                 bud[B_DET] += (double)(dt * (e1 + e3));
                 if (fail) {
                     zn_c = zn_c - (double)(R(0.99) * m * kbed);
@@ -891,7 +934,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
                 }
             }
             const R dh = (u - chtemp) * kbed;
-            hn = hn + dh;
+            if constexpr (sizeof(R) == sizeof(double)) hn = hn + dh;
+            else dh_sum += dh;
             bud[B_EXCH] += (double)dh;
             totm += m;
             depn += m * inv_rhos;
@@ -927,7 +971,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         }
         // second chunk, classes KS .. S-1: their upwind concentrations are staged as in
         // phase 1 over the planes from s_u on, which no thread reads after this point
-        if (SPLIT && S > KS) {
+        if (TWO_CHUNK && S > KS) {
             __syncthreads();
             R *const s_sed1 = s_u;
             constexpr int KB = 2;  // classes per batch of halo loads
@@ -993,6 +1037,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
                 update_class(k, ch_k, m_k, s_sed1 + (k - KS) * EXT);
             }
         }
+
+        if constexpr (sizeof(R) == sizeof(float)) hn = add_unbiased(hn_pre, dh_sum, next_bits());
 
         // dry-cell clamp (SWEHR.c:1532-1551)
         if (!(hn > hfilm) && own) {

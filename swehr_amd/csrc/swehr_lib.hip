@@ -69,18 +69,30 @@ void launch_step_v(int id, int blocks, size_t lds, hipStream_t s, const StepArgs
     }
 }
 
-// the variant that stages the classes in two chunks only where there are more
-// classes than LDS planes for them
+// Waves per SIMD that the LDS of a block of `threads` admits: 160 KB per CU shared by
+// its 4 SIMDs, with 1 KB of static LDS per block (the step kernel's tables) beside the
+// dynamic size.
+inline int lds_waves_per_simd(size_t lds, int threads)
+{
+    return (int)((163840 / (lds + 1024)) * (size_t)(threads / 64) / 4);
+}
+
+// The variant for many classes (SPLIT) where the default one does not serve them
+// well: in FP64 where there are more classes than LDS planes for them (two chunks),
+// in FP32 where the tile's LDS admits fewer waves per SIMD than the 5 the default
+// variant is compiled for (the SPLIT variant's 4 then cost nothing).
+template <typename R>
+bool use_split(int id, int S, size_t lds)
+{
+    if constexpr (kLdsClasses<R> < kMaxClasses) return S > kLdsClasses<R>;
+    else return lds_waves_per_simd(lds, kTiles[id - 1].ti * kTiles[id - 1].tj) < kStepWaves<R, false>;
+}
+
 template <typename R>
 void launch_step(int id, int blocks, size_t lds, hipStream_t s, const StepArgs<R> &A)
 {
-    if constexpr (kLdsClasses<R> < kMaxClasses) {
-        if (A.S > kLdsClasses<R>) {
-            launch_step_v<R, true>(id, blocks, lds, s, A);
-            return;
-        }
-    }
-    launch_step_v<R, false>(id, blocks, lds, s, A);
+    if (use_split<R>(id, A.S, lds)) launch_step_v<R, true>(id, blocks, lds, s, A);
+    else launch_step_v<R, false>(id, blocks, lds, s, A);
 }
 
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
@@ -135,13 +147,14 @@ void launch_batch_wave(int id, int prec, int blocks, hipStream_t s, const void *
 void launch_batch_step(int id, int prec, int blocks, size_t lds, hipStream_t s, const void *args, const TileRef *map,
                        int n, int per_xcd, int max_classes)
 {
-    static_assert(kLdsClasses<float> == kMaxClasses, "FP32 has no two-chunk variant");
     if (prec == SWEHR_FP64) {
         const StepArgs<double> *a = (const StepArgs<double> *)args;
-        if (max_classes > kLdsClasses<double>) batch_step_t<double, true>(id, blocks, lds, s, a, map, n, per_xcd);
+        if (use_split<double>(id, max_classes, lds)) batch_step_t<double, true>(id, blocks, lds, s, a, map, n, per_xcd);
         else batch_step_t<double, false>(id, blocks, lds, s, a, map, n, per_xcd);
     } else {
-        batch_step_t<float, false>(id, blocks, lds, s, (const StepArgs<float> *)args, map, n, per_xcd);
+        const StepArgs<float> *a = (const StepArgs<float> *)args;
+        if (use_split<float>(id, max_classes, lds)) batch_step_t<float, true>(id, blocks, lds, s, a, map, n, per_xcd);
+        else batch_step_t<float, false>(id, blocks, lds, s, a, map, n, per_xcd);
     }
 }
 

@@ -97,6 +97,50 @@ __device__ __forceinline__ double exp_tab(double x, const double *tab)
     return __builtin_ldexp(tab[j] * p, m);
 }
 __device__ __forceinline__ float mexp(float x) { return expf(x); }
+// exp(x) - 1 for x <= 0 without the cancellation of the difference at small |x| (FP32
+// settling): the Taylor series to x^6 for x > -1/8 (truncation below 1e-9 relative),
+// expf(x) - 1 below, where the difference is at least 0.117 and keeps its digits
+__device__ __forceinline__ float mexpm1_neg(float x)
+{
+    if (x > -0.125f) {
+        float p = __builtin_fmaf(x, 1.0f / 720, 1.0f / 120);
+        p = __builtin_fmaf(x, p, 1.0f / 24);
+        p = __builtin_fmaf(x, p, 1.0f / 6);
+        p = __builtin_fmaf(x, p, 0.5f);
+        p = __builtin_fmaf(x, p, 1.0f);
+        return x * p;
+    }
+    return expf(x) - 1.0f;
+}
+
+// 32 well-mixed bits of x (the finaliser of MurmurHash3)
+__device__ __forceinline__ unsigned mix32(unsigned x)
+{
+    x ^= x >> 16;
+    x *= 0x85ebca6bu;
+    x ^= x >> 13;
+    x *= 0xc2b2ae35u;
+    x ^= x >> 16;
+    return x;
+}
+
+// a + b rounded to a float without bias (stochastic rounding): to the float below or
+// above the exact sum with probabilities in proportion to its distance from the other
+// one, decided by the 32 random bits `bits`.  The expected result is the exact sum, so
+// an increment smaller than half the spacing of floats at a still accumulates over many
+// steps, where rounding to nearest would drop it every time.  The exact error of the
+// rounded sum comes from Knuth's TwoSum (every operation rounded once, no contraction).
+__device__ __forceinline__ float add_unbiased(float a, float b, unsigned bits)
+{
+    const float s = a + b;
+    const float bv = s - a;
+    const float e = (a - (s - bv)) + (b - bv);  // a + b - s, exactly
+    if (!(e != 0.0f) || s == 0.0f) return s;  // exact (or NaN)
+    const int k = __float_as_int(s);
+    const float nb = __int_as_float((e > 0.0f) == (s > 0.0f) ? k + 1 : k - 1);  // s's neighbour on e's side
+    const float gap = nb - s;  // exact
+    return (float)(bits >> 8) * 0x1p-24f * __builtin_fabsf(gap) < __builtin_fabsf(e) ? nb : s;
+}
 
 // For F = 1 + j/32, j = -9..13: 1/F rounded, and ln F as a rounded head plus the
 // rounded rest (the table of log_tab, staged in LDS).
