@@ -800,7 +800,18 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         const int fxl = il * TJ + jl, fxr = fxl + TJ;
         const int fyl = il * (TJ + 1) + jl, fyr = fyl + 1;
         const R dx_ = dt * odx;
-        R hn = h - dx_ * (s_fx[fxr] - s_fx[fxl]) - dx_ * (s_fy[fyr] - s_fy[fyl]) + dt * Rr;
+        // FP32: the depth's increments of the step (fluxes, rain, infiltration, exchange)
+        // are summed apart in dh_rest and added to the depth h_base once, without bias,
+        // after the class loop.  In deep water they are a few float spacings at h each
+        // step (rain 1.3e-6 m against 9.5e-7 m at 8 m), and rounded to nearest with the
+        // same fraction every step they would gain or lose water systematically.
+        R hn, h_base = h, dh_rest = R(0);
+        if constexpr (sizeof(R) == sizeof(double)) {
+            hn = h - dx_ * (s_fx[fxr] - s_fx[fxl]) - dx_ * (s_fy[fyr] - s_fy[fyl]) + dt * Rr;
+        } else {
+            dh_rest = dt * Rr - dx_ * (s_fx[fxr] - s_fx[fxl]) - dx_ * (s_fy[fyr] - s_fy[fyl]);
+            hn = h + dh_rest;
+        }
         R hun = hu - dx_ * (s_fx[NFX + fxr] - s_fx[NFX + fxl]) - dx_ * (s_fy[NFY + fyr] - s_fy[NFY + fyl]) + dt * sxs +
                 dt * gmx + dt * dfm * btx;
         R hvn = hv - dx_ * (s_fx[2 * NFX + fxr] - s_fx[2 * NFX + fxl]) -
@@ -813,13 +824,21 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             const double dth = A.dtheta_raster ? at(gmem(A.dtheta_raster), c8L) : A.thetas - A.theta0;
             const double hf_c = A.hf_raster ? at(gmem(A.hf_raster), c8L) : A.hf;
             const double zf = vin * mrcp(dth);
-            const double hd = (double)hn;
+            const double hd = sizeof(R) == sizeof(double) ? (double)hn : (double)h + (double)dh_rest;
             // the film depth as stored in this precision, so a dry cell stays exactly dry
             // zf = 0 gives NaN where division gives inf (or NaN): fmin returns h - hfilm for both
             const double infl = mfmin(hd - (double)hfilm, dtd * ks * (zf + hf_c + hd) * mrcp(zf));
             const double vin_new = vin + infl;
             if (own && !same_bits(vin_new, vin)) st_stream(vinfp, c8, vin_new);
             hn = R(hd - infl);
+            if constexpr (sizeof(R) == sizeof(float)) {
+                if (infl == hd - (double)hfilm) {  // the film clamp: exactly the film depth
+                    h_base = hfilm;
+                    dh_rest = R(0);
+                } else {
+                    dh_rest = dh_rest - R(infl);
+                }
+            }
         }
 
         // sediment: flux divergence, sources and exchange per class (SWEHR.c:1454-1511)
@@ -837,9 +856,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         // FP32: the exchange with the bed changes h by ~1e-10 m per class and step, far
         // below the spacing of floats at h (9.3e-10 m at 1 cm) and mostly of one sign, so
         // added to h one by one it would be lost every step (a systematic water budget
-        // error).  The classes' increments are also summed apart, and their sum is added
-        // to the depth from before the class loop once, with one unbiased rounding.
-        const R hn_pre = hn;
+        // error).  The classes' increments are summed apart and join dh_rest.
         R dh_sum = R(0);
         // random bits of the unbiased roundings (FP32): a hash of the cell and the step,
         // then a linear congruential sequence over the values the cell stores, so results
@@ -891,8 +908,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             if (((A.coarse2 >> k) & 1u) && shallow_v) vf = R(2e-3) + vshal * (vf - R(2e-3));
             vf = vf * vf_keep;
             // exchange with the deposited layer
-            // the running depth: in FP32 the depth before the loop plus the increments so far
-            const R hcur = sizeof(R) == sizeof(double) ? hn : hn_pre + dh_sum;
+            // the running depth: in FP32 the depth before the step plus the increments so far
+            const R hcur = sizeof(R) == sizeof(double) ? hn : h_base + (dh_rest + dh_sum);
             if (hcur < R(10) * hfilm) {
                 if constexpr (sizeof(R) == sizeof(double)) m += dt * e1 + u;
                 else m = add_unbiased(m, dt * e1 + u, next_bits());
@@ -926,10 +943,20 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
                 }
                 bud[B_DET] += (double)(dt * (e1 + e3));
                 if (fail) {
-                    zn_c = zn_c - (double)(R(0.99) * m * kbed);
-                    u = u + R(0.99) * m;
+                    if constexpr (sizeof(R) == sizeof(double)) {
+                        zn_c = zn_c - (double)(R(0.99) * m * kbed);
+                        u = u + R(0.99) * m;
+                        m = R(0.01) * m;
+                    } else {
+                        // FP32: the float 0.99 and 0.01 do not sum to 1, so the two
+                        // products would not conserve the class's mass; the mass kept is
+                        // rounded once and the rest moves into suspension
+                        const R keep = R(0.01) * m, moved = m - keep;
+                        zn_c = zn_c - (double)(moved * kbed);
+                        u = add_unbiased(u, moved, next_bits());
+                        m = keep;
+                    }
                     totm = R(0);
-                    m = R(0.01) * m;
                     depn = R(0);
                 }
             }
@@ -1038,7 +1065,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             }
         }
 
-        if constexpr (sizeof(R) == sizeof(float)) hn = add_unbiased(hn_pre, dh_sum, next_bits());
+        if constexpr (sizeof(R) == sizeof(float)) hn = add_unbiased(h_base, dh_rest + dh_sum, next_bits());
 
         // dry-cell clamp (SWEHR.c:1532-1551)
         if (!(hn > hfilm) && own) {
@@ -1059,7 +1086,9 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
                     mm_a = at(Mp + (k + 1) * plane, cR);
                 }
                 zn_c = zn_c + (double)(kbed * u);
-                at(Mp + k * plane, cR) = mm + u;
+                // FP32: u is often far below the float spacing at mm
+                if constexpr (sizeof(R) == sizeof(double)) at(Mp + k * plane, cR) = mm + u;
+                else at(Mp + k * plane, cR) = add_unbiased(mm, u, next_bits());
                 at(chq + k * plane, cR) = R(0);
             }
         }

@@ -99,7 +99,9 @@ __device__ __forceinline__ double exp_tab(double x, const double *tab)
 __device__ __forceinline__ float mexp(float x) { return expf(x); }
 // exp(x) - 1 for x <= 0 without the cancellation of the difference at small |x| (FP32
 // settling): the Taylor series to x^6 for x > -1/8 (truncation below 1e-9 relative),
-// expf(x) - 1 below, where the difference is at least 0.117 and keeps its digits
+// below that the hardware exp2 minus 1, where the difference is at least 0.117 and
+// keeps its digits.  The lanes of a wave update the same class, so their x are alike
+// and the branch is mostly uniform.
 __device__ __forceinline__ float mexpm1_neg(float x)
 {
     if (x > -0.125f) {
@@ -110,7 +112,7 @@ __device__ __forceinline__ float mexpm1_neg(float x)
         p = __builtin_fmaf(x, p, 1.0f);
         return x * p;
     }
-    return expf(x) - 1.0f;
+    return __builtin_amdgcn_exp2f(x * 1.44269504f) - 1.0f;
 }
 
 // 32 well-mixed bits of x (the finaliser of MurmurHash3)
@@ -135,9 +137,11 @@ __device__ __forceinline__ float add_unbiased(float a, float b, unsigned bits)
     const float s = a + b;
     const float bv = s - a;
     const float e = (a - (s - bv)) + (b - bv);  // a + b - s, exactly
-    if (!(e != 0.0f) || s == 0.0f) return s;  // exact (or NaN)
+    // s's neighbour on e's side: one unit more in magnitude where e has s's sign, one
+    // less otherwise.  With e = 0 (exact sum, and any zero sum) the test below fails
+    // and s is returned; a NaN or infinite s makes e NaN, which fails it as well.
     const int k = __float_as_int(s);
-    const float nb = __int_as_float((e > 0.0f) == (s > 0.0f) ? k + 1 : k - 1);  // s's neighbour on e's side
+    const float nb = __int_as_float(k + (((__float_as_int(e) ^ k) >> 31) | 1));
     const float gap = nb - s;  // exact
     return (float)(bits >> 8) * 0x1p-24f * __builtin_fabsf(gap) < __builtin_fabsf(e) ? nb : s;
 }
