@@ -590,7 +590,8 @@ def main():
     wres = sv.water_residual(bud)
     wscale = max(abs(bud[k]) for k in (0, 1, 2, 3, 9, 10))
     out = {
-        "metric": "cell-class-steps/s (cells x size classes x steps per second), plain FP64 step",
+        "metric": "cell-class-steps/s (cells x size classes x steps per second), plain %s step"
+                  % args.precision.upper(),
         "value": value,
         "unit": "cell-class-steps/s",
         "n_gpus": 1,

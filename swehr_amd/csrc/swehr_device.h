@@ -801,11 +801,12 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         const int fyl = il * (TJ + 1) + jl, fyr = fyl + 1;
         const R dx_ = dt * odx;
         // FP32: the depth's increments of the step (fluxes, rain, infiltration, exchange)
-        // are summed apart in dh_rest and added to the depth h_base once, without bias,
+        // are summed apart in dh_rest and added to the depth once, without bias,
         // after the class loop.  In deep water they are a few float spacings at h each
         // step (rain 1.3e-6 m against 9.5e-7 m at 8 m), and rounded to nearest with the
         // same fraction every step they would gain or lose water systematically.
-        R hn, h_base = h, dh_rest = R(0);
+        R hn, dh_rest = R(0);
+        bool film = false;  // FP32: the infiltration term clamped the cell to the film depth
         if constexpr (sizeof(R) == sizeof(double)) {
             hn = h - dx_ * (s_fx[fxr] - s_fx[fxl]) - dx_ * (s_fy[fyr] - s_fy[fyl]) + dt * Rr;
         } else {
@@ -833,7 +834,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             hn = R(hd - infl);
             if constexpr (sizeof(R) == sizeof(float)) {
                 if (infl == hd - (double)hfilm) {  // the film clamp: exactly the film depth
-                    h_base = hfilm;
+                    film = true;
                     dh_rest = R(0);
                 } else {
                     dh_rest = dh_rest - R(infl);
@@ -856,8 +857,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         // FP32: the exchange with the bed changes h by ~1e-10 m per class and step, far
         // below the spacing of floats at h (9.3e-10 m at 1 cm) and mostly of one sign, so
         // added to h one by one it would be lost every step (a systematic water budget
-        // error).  The classes' increments are summed apart and join dh_rest.
-        R dh_sum = R(0);
+        // error).  The classes' increments join dh_rest.
         // random bits of the unbiased roundings (FP32): a hash of the cell and the step,
         // then a linear congruential sequence over the values the cell stores, so results
         // are reproducible and do not depend on the tile shape or on the ensemble
@@ -909,7 +909,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             vf = vf * vf_keep;
             // exchange with the deposited layer
             // the running depth: in FP32 the depth before the step plus the increments so far
-            const R hcur = sizeof(R) == sizeof(double) ? hn : h_base + (dh_rest + dh_sum);
+            const R hcur = sizeof(R) == sizeof(double) ? hn : (film ? hfilm : h) + dh_rest;
             if (hcur < R(10) * hfilm) {
                 if constexpr (sizeof(R) == sizeof(double)) m += dt * e1 + u;
                 else m = add_unbiased(m, dt * e1 + u, next_bits());
@@ -962,7 +962,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             }
             const R dh = (u - chtemp) * kbed;
             if constexpr (sizeof(R) == sizeof(double)) hn = hn + dh;
-            else dh_sum += dh;
+            else dh_rest += dh;
             bud[B_EXCH] += (double)dh;
             totm += m;
             depn += m * inv_rhos;
@@ -1065,7 +1065,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             }
         }
 
-        if constexpr (sizeof(R) == sizeof(float)) hn = add_unbiased(h_base, dh_rest + dh_sum, next_bits());
+        if constexpr (sizeof(R) == sizeof(float)) hn = add_unbiased(film ? hfilm : h, dh_rest, next_bits());
 
         // dry-cell clamp (SWEHR.c:1532-1551)
         if (!(hn > hfilm) && own) {
