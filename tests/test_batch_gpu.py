@@ -71,18 +71,21 @@ def test_ensemble_members_equal_alone_runs(precision):
         assert len({b.member(i).info()["steps"] for i in range(len(cases))}) > 1
 
 
-def test_ensemble_mixed_class_counts_equal_alone_runs():
-    """FP64 members with more size classes than the step kernel has LDS planes for
-    (the launch then takes the two-chunk variant for every member) beside members with
-    fewer: each equals its own run alone, which takes the variant its S selects."""
+@pytest.mark.parametrize("precision", [_abi.FP64, _abi.FP32], ids=["fp64", "fp32"])
+def test_ensemble_mixed_class_counts_equal_alone_runs(precision):
+    """Members with many size classes beside members with few.  The launch takes the
+    variant for many classes for every member (FP64: two LDS chunks; FP32: 4 waves per
+    SIMD with the own cell's classes in registers); each member equals its own run
+    alone, which takes the variant its S selects.  In FP32 this also checks that the
+    unbiased roundings draw the same random bits in both."""
     from parity_util import many_classes_case
 
     cases = [many_classes_case(10, 37, 53, seed=1), _golden("transmissive"), many_classes_case(16, 21, 70, seed=2),
              _golden("closed_basin")]
-    with sv.Batch(cases, precision=_abi.FP64) as b:
+    with sv.Batch(cases, precision=precision) as b:
         b.advance(300)
         for i, c in enumerate(cases):
-            with sv.Solver(c, precision=_abi.FP64) as s:
+            with sv.Solver(c, precision=precision) as s:
                 s.run(300)
                 assert s.info()["steps"] == b.member(i).info()["steps"], c.name
                 _same(b.member(i), s, c)
