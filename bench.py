@@ -25,6 +25,7 @@ import copy
 import ctypes as C
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -143,7 +144,6 @@ def reference_binary_baseline(case, budget_s: float, step_s_estimate: float):
     times is n steps, so parsing, set-up and the final raster writes cancel.  Returns
     (n, seconds, seconds of the 1-step run), or None where the build is absent."""
     import shutil
-    import subprocess
     import tempfile
 
     binary = os.path.join(ROOT, "oracle", "_ref", "swehr_ref_hooked")
@@ -563,7 +563,11 @@ def main():
                "host": host_description(),
                "parity_c1": parity_c1(threads)}
         if not args.no_ref_binary:
-            rb = reference_binary_baseline(case, args.cpu_seconds / 2, el1 / n1)
+            try:
+                rb = reference_binary_baseline(case, args.cpu_seconds / 2, el1 / n1)
+            except (OSError, RuntimeError, ValueError, subprocess.SubprocessError) as ex:  # a baseline, not the product
+                rb = None
+                cpu["reference_binary"] = {"error": str(ex)[:300]}
             if rb is not None:
                 nr, elr, t_one = rb
                 cpu["reference_binary"] = {
