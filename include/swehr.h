@@ -46,7 +46,11 @@ enum swehr_status {
 
 enum swehr_precision {
     SWEHR_FP64 = 0,          /* plain: all state and arithmetic in FP64, like the reference */
-    SWEHR_FP32 = 1           /* state and fluxes FP32; dt, cumulative infiltration and budgets FP64 */
+    SWEHR_FP32 = 1           /* state and fluxes FP32; dt, bed elevation, cumulative infiltration
+                                and budgets FP64.  Increments below the float spacing of the
+                                value they change are added with unbiased rounding, whose random
+                                bits hash the cell and the step: runs are reproducible, and an
+                                instance gives the same bits alone and in an ensemble or server */
 };
 
 /* Field ids for swehr_download().  k is the 1-based size class for per-class fields. */
