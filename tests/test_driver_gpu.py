@@ -100,3 +100,25 @@ def test_driver_reads_parameter_raster_files(tmp_path):
     for fn in ("depth.txt", "uh.txt", "vh.txt", "ch.txt"):
         va, vb = _numbers(str(tmp_path / "oracle" / fn)), _numbers(str(tmp_path / "gpu" / fn))
         np.testing.assert_allclose(vb, va, rtol=1e-7, atol=1e-8, err_msg=fn)
+
+
+@pytest.mark.parametrize("name", ["transmissive", "closed_basin"])
+def test_driver_fp32_precision_form(name, tmp_path):
+    """Sidecar "precision = fp32" runs the drop-in program in the precision form: the
+    same files and stdout lines as the oracle's FP64 program, with every raster within
+    FP32 accuracy (the oracle's program reads the key and stays FP64)."""
+    src = str(tmp_path / "in")
+    shutil.copytree(os.path.join(GOLD, name, "inputs"), src)
+    with open(os.path.join(src, "swehr_ext.txt"), "a") as fh:
+        fh.write("precision = fp32\n")
+    a = _run(RUN_PATH, src, str(tmp_path / "oracle"))
+    b = _run(DRIVER, src, str(tmp_path / "gpu"))
+    assert len(a) == len(b)
+    for fn in ("topoout.txt", "depth.txt", "uh.txt", "vh.txt", "ch.txt", "m.txt", "c.txt", "vel.txt",
+               "maxvel.txt", "maxdepth.txt"):
+        va, vb = _numbers(str(tmp_path / "oracle" / fn)), _numbers(str(tmp_path / "gpu" / fn))
+        assert va.shape == vb.shape, fn
+        scale = max(float(np.max(np.abs(va))), 1e-12)
+        err = float(np.max(np.abs(vb - va))) / scale
+        print(fn, "%.2e" % err)
+        assert err <= 1e-4, fn
