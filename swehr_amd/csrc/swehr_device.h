@@ -859,8 +859,10 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         // added to h one by one it would be lost every step (a systematic water budget
         // error).  The classes' increments join dh_rest.
         // random bits of the unbiased roundings (FP32): a hash of the cell and the step,
-        // then a linear congruential sequence over the values the cell stores, so results
-        // are reproducible and do not depend on the tile shape or on the ensemble
+        // then a linear congruential sequence (the Numerical Recipes multiplier and
+        // increment, mod 2^32; the high bits are used) over the values the cell stores,
+        // so results are reproducible and do not depend on the tile shape or on the
+        // ensemble
         unsigned rbits = sizeof(R) == sizeof(float) ? mix32(c ^ mix32((unsigned)steps)) : 0u;
         auto next_bits = [&]() {
             rbits = rbits * 1664525u + 1013904223u;
