@@ -212,6 +212,38 @@ def test_fp64_many_classes_match_oracle(S, nx, ny):
     o.close()
 
 
+@pytest.mark.parametrize("precision,tol", [(_abi.FP64, RTOL_FP64_SHORT), (_abi.FP32, 1e-4)], ids=["fp64", "fp32"])
+def test_channel_network_case_matches_oracle(precision, tol):
+    """The inputs of the 4096^2 precision-form case at 72 x 56 (ragged against the tiles):
+    fractal surface with carved channels, a per-cell Ks raster, 10 classes with
+    Dirichlet fractions and a rain table of 10 s entries (here falling from the
+    hyetograph's peak, so that the surface runs off within the steps compared): FP64 to
+    rounding level, FP32 within 1e-4 on every field, after 300 steps.  Some 40 steps
+    later a receding cell meets the film-depth threshold at a last-bit difference: from
+    there the oracle itself, with Ks perturbed by 1e-15 relative, moves by 8e-4 rel L2
+    at 350 steps (the GPU by 6e-4), so the comparison stops before."""
+    from swehr_amd.inputs import carve_channels, config_c3, fractal_topography
+
+    case = config_c3(seed=21, n=64)
+    rng = np.random.default_rng(21)
+    case.nx, case.ny = 72, 56
+    case.topo = carve_channels(fractal_topography(72, 56, 1.0, 20.0, 0.05, 0.8, rng), 1.0, rng)
+    case.ks_raster = 2.7778e-6 * np.exp(0.5 * rng.standard_normal((72, 56)))
+    case.rain = float(case.rain.max()) * (1.0 - np.arange(len(case.rain)) / len(case.rain))
+    print("channel case", case.nx, case.ny, "precision", precision, flush=True)
+    o, s = _pair(case, 300, precision=precision)
+    assert float(o.state()["depth"].mean()) > 5e-5  # the surface is wet (film depth 1e-5) and running off
+    errs = compare_states(s.state(), o.state(), case.nclass)
+    print({k: "%.2e" % v for k, v in errs.items()})
+    k, e = worst(errs)
+    assert e <= tol, (k, e)
+    st_g, _ = s.stage()
+    st_o, _ = o.stage()
+    assert rel_l2(st_g, st_o) <= tol
+    s.close()
+    o.close()
+
+
 def test_fp32_sixteen_classes_close_to_oracle():
     """FP32 form at the maximum of 16 classes (all of them in LDS planes of the tile)."""
     from parity_util import many_classes_case
