@@ -2,13 +2,17 @@
 
 Default workload (the plain FP64 configuration): 2048 x 2048 cells, dx 1 m, fractal
 topography, 7 size classes, 80 mm/h rain for 1800 s, per-cell Ks.  A "step" is one
-coupled time step over the whole grid.  After ``--warmup`` untimed steps (the default
-reaches ~600 s of simulated time, when runoff is established) exactly ``--steps`` steps
-are timed with HIP events on the solver's stream, bracketed by device
-synchronisations.  ``value`` is cell-class-steps/s = cells x size classes x steps / s.
+coupled time step over the whole grid.  After at least ``--warmup`` untimed steps, and
+in any case not before ``--t-start`` (600 s) of simulated time, when runoff is
+established, exactly ``--steps`` steps are timed with HIP events on the solver's
+stream, bracketed by device synchronisations; ``config.t_window_s`` gives the
+simulated times the window spans.  ``value`` is cell-class-steps/s = cells x size
+classes x steps / s.
 
-Roofline: the fused step kernel's average duration is measured live (HIP events around
-eagerly launched steps); its algorithmic HBM bytes per launch are cells x B_cell with
+Roofline: the fused step kernel's average duration is measured live over the same
+steps as the timed window (the window's steps are launched one by one with HIP events
+around each kernel, the state is put back from a device copy, then the window is timed
+as graph launches); its algorithmic HBM bytes per launch are cells x B_cell with
 B_cell = 2 w (6 + 2 S) + 16 + w bytes (one read and one write of h, hu, hv, z, MAXDEPTH,
 MAXVEL, ch_k, M_k at w bytes, VINF as FP64, one read of the Ks raster).  The FP32 form
 keeps z and the Ks raster in FP64, 12 B per cell more than this count credits it with.  ``traffic`` is
@@ -115,6 +119,24 @@ def timed_run(hip, solver, steps):
     wall = time.perf_counter() - t0
     ms = hip.elapsed_ms(e0, e1)
     return info["steps"] - i0["steps"], ms, wall, info
+
+
+def graph_length(steps: int, most: int) -> int:
+    """Steps per captured graph for a window of `steps`: the largest divisor of steps
+    not above max(most, 64), so the window is whole graphs."""
+    top = max(1, max(most, 64))
+    return max(d for d in range(1, min(steps, top) + 1) if steps % d == 0)
+
+
+def advance_to(solver, t_target: float, maxstep: float):
+    """Step until the simulated time reaches t_target.  A step is at most maxstep long,
+    so ceil((t_target - t) / maxstep) steps never overshoot by more than one step."""
+    import math
+
+    info = solver.info()
+    while info["t"] < t_target and not info["done"]:
+        info = solver.run(max(1, int(math.ceil((t_target - info["t"]) / maxstep))))
+    return info
 
 
 def cpu_baseline(case, budget_s: float, threads: int):
@@ -460,7 +482,11 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1200)
-    ap.add_argument("--warmup", type=int, default=2400)
+    ap.add_argument("--warmup", type=int, default=2400,
+                    help="untimed steps before the timed window (a minimum: the window always starts at "
+                         "--t-start simulated seconds or later)")
+    ap.add_argument("--t-start", type=float, default=600.0,
+                    help="simulated time at which the timed window starts at the earliest (runoff established)")
     ap.add_argument("--grid", type=int, default=0, help="grid side (default 2048, or 4096 with --case c3)")
     ap.add_argument("--case", choices=["c2", "c3"], default="c2",
                     help="headline workload: the 2048^2, S=7 fractal plane (c2, the metric's configuration) or "
@@ -471,7 +497,6 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ref-binary", action="store_true",
                     help="skip timing the compiled reference (oracle/_ref) beside the OpenMP port")
-    ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--steps-per-graph", type=int, default=16)
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--forms", default="precision,batched,serving",
@@ -524,16 +549,26 @@ def main():
     s = sv.Solver(case, precision=prec, steps_per_graph=args.steps_per_graph, tile=args.tile)
     # same window as the CPU baseline: the first steps from the initial state
     n_first, ms_first, _, _ = timed_run(hip, s, 20)
+    # --warmup is a minimum: the timed window starts once runoff is established
+    # (t >= --t-start simulated seconds), whatever the step count asked for
     s.run(max(0, args.warmup - n_first))
-    log("warmup done: t=%.1f s after %d steps" % (s.info()["t"], s.info()["steps"]))
+    advance_to(s, args.t_start, case.side("maxstep"))
+    t_start = s.info()["t"]
+    log("warmup done: t=%.1f s after %d steps" % (t_start, s.info()["steps"]))
     tile_ms = None
     if not args.tile:
         best, tile_ms = s.autotune(8)
         log("autotuned tile %d: %s ms per step kernel" % (best, [round(float(x), 4) for x in tile_ms]))
+    # the step kernel's duration over exactly the steps of the timed window: the window's
+    # steps are launched one by one with events around each kernel, the state is put
+    # back from a device copy, and the same steps are timed as graph launches (captured
+    # in graphs whose length divides the window, so no graph runs past its end)
+    kms, taken = s.profile_window(args.steps)
+    s.configure(steps_per_graph=graph_length(args.steps, args.steps_per_graph))
     n, ms, wall, info = timed_run(hip, s, args.steps)
     value = cells * S * n / (ms * 1e-3)
-    log("timed %d steps: %.3f ms (wall %.3f s) -> %.4e cell-class-steps/s, t=%.1f" % (n, ms, wall, value, info["t"]))
-    kms, taken = s.profile(args.profile_steps)
+    log("timed %d steps: %.3f ms (wall %.3f s) -> %.4e cell-class-steps/s, t=%.1f..%.1f"
+        % (n, ms, wall, value, t_start, info["t"]))
     bud = s.budget()
     s.close()
 
@@ -612,7 +647,7 @@ def main():
                                 "plain %s step, %dx%d cells, dx 1 m, channel network, S=%d, triangular hyetograph, "
                                 "per-cell Ks") % (args.precision.upper(), args.grid, args.grid, S),
                    "grid": [case.nx, case.ny], "classes": S, "precision": args.precision,
-                   "t_end_s": info["t"], "tile": args.tile or "autotuned",
+                   "t_window_s": [round(t_start, 3), round(info["t"], 3)], "tile": args.tile or "autotuned",
                    "autotune_step_kernel_ms": None if tile_ms is None else [round(float(x), 4) for x in tile_ms]},
         "roofline": roofline,
         "cpu_baseline": cpu,

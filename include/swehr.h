@@ -211,10 +211,21 @@ int swehr_get_stream(swehr_handle *h, void **stream);
  * kernels in avg_ms[0..2]; *taken receives the number of steps run. */
 int swehr_profile(swehr_handle *h, int nsteps, double *avg_ms, int *taken);
 
+/* As swehr_profile, then the instance's state is put back (from a device copy taken
+ * before the first profiled step), so the steps profiled are exactly the ones the next
+ * swehr_advance takes.  Diagnostic; no counterpart in the reference. */
+int swehr_profile_window(swehr_handle *h, int nsteps, double *avg_ms, int *taken);
+
 /* Checkpoint of the instance's whole device state (snapshot/restore): *bytes is the
  * snapshot size; swehr_snapshot copies the state into host memory of that size, and
  * swehr_restore loads a snapshot taken from an instance created with the same
- * parameters.  No counterpart in the reference (its final text rasters are lossy). */
+ * parameters.  The snapshot starts with a header of the instance's layout (grid,
+ * classes, precision, Ks raster, run length, which parameter rasters are set); a
+ * restore into an instance of another layout is refused with SWEHR_ERR_BAD_INPUT.
+ * The constant parameter rasters (swehr_set_parameter_raster) are not part of a
+ * snapshot: set them on the target instance before restoring.  The host buffer must
+ * hold swehr_snapshot_bytes() bytes.  No counterpart in the reference (its final text
+ * rasters are lossy). */
 int swehr_snapshot_bytes(swehr_handle *h, int64_t *bytes);
 int swehr_snapshot(swehr_handle *h, void *host);
 int swehr_restore(swehr_handle *h, const void *host);

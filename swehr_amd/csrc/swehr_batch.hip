@@ -107,10 +107,14 @@ struct Engine {
         return SWEHR_OK;
     }
 
+    // Grow the member tables and the work map.  The capacities are zeroed before the old
+    // buffers are freed, so an allocation that fails leaves capacity 0 and the next
+    // rebuild allocates again instead of using buffers that no longer exist.
     int reserve(size_t members_needed, size_t map_needed)
     {
         if (members_needed > cap_members) {
             size_t c = std::max<size_t>(members_needed, 2 * cap_members);
+            cap_members = 0;
             if (d_args) (void)hipFree(d_args);
             if (d_insts) (void)hipFree(d_insts);
             if (d_ctrls) (void)hipFree(d_ctrls);
@@ -130,6 +134,7 @@ struct Engine {
         }
         if (map_needed > cap_map) {
             size_t c = std::max<size_t>(map_needed, 2 * cap_map);
+            cap_map = 0;
             if (d_map) (void)hipFree(d_map);
             d_map = nullptr;
             HIPCHK(hipMalloc((void **)&d_map, c * sizeof(TileRef)));

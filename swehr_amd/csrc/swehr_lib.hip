@@ -777,10 +777,62 @@ extern "C" int swehr_profile(swehr_handle *h, int nsteps, double *avg_ms, int *t
                                       : profile_impl(h, h->a32, nsteps, avg_ms, taken);
 }
 
+// A snapshot is a header that records the instance's layout, then its whole device
+// block (which holds no pointers, so it restores into any instance of that layout).
+namespace {
+constexpr uint64_t kSnapMagic = 0x3150414e53524857ull;  // "WHRSNAP1"
+constexpr size_t kSnapHeader = 256;
+struct SnapHeader {
+    uint64_t magic;
+    int64_t bytes;  // device block bytes that follow the header
+    int32_t nx, ny, S, precision, ks_raster, rasters, stage_cap, bud_slots;
+};
+static_assert(sizeof(SnapHeader) <= kSnapHeader, "snapshot header");
+
+SnapHeader snap_header(const swehr_handle *h)
+{
+    SnapHeader s;
+    memset(&s, 0, sizeof s);
+    s.magic = kSnapMagic;
+    s.bytes = (int64_t)h->bytes;
+    s.nx = h->nx;
+    s.ny = h->ny;
+    s.S = h->S;
+    s.precision = h->precision;
+    s.ks_raster = (h->precision == SWEHR_FP64 ? h->a64.ks_raster : h->a32.ks_raster) != nullptr;
+    for (int k = 0; k < 3; k++)
+        if (h->prast[k]) s.rasters |= 1 << k;
+    s.stage_cap = h->stage_cap;
+    s.bud_slots = h->bud_slots;
+    return s;
+}
+}  // namespace
+
+extern "C" int swehr_profile_window(swehr_handle *h, int nsteps, double *avg_ms, int *taken)
+{
+    if (!h || !avg_ms || nsteps < 0) return SWEHR_ERR_BAD_INPUT;
+    HIPCHK(hipSetDevice(h->device));
+    void *save = nullptr;
+    if (hipMalloc(&save, h->bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        swehr_set_error("no device memory for the profiler's state copy%s%s", "", "");
+        return SWEHR_ERR_OOM;
+    }
+    int rc = SWEHR_OK;
+    if (hipMemcpyAsync(save, h->mem, h->bytes, hipMemcpyDeviceToDevice, h->stream) != hipSuccess) rc = SWEHR_ERR_HIP;
+    if (!rc) rc = swehr_profile(h, nsteps, avg_ms, taken);
+    if (hipMemcpyAsync(h->mem, save, h->bytes, hipMemcpyDeviceToDevice, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess)
+        rc = rc ? rc : SWEHR_ERR_HIP;
+    (void)hipFree(save);
+    if (rc == SWEHR_ERR_HIP && !g_err[0]) swehr_set_error("profile window: %s%s", hipGetErrorString(hipGetLastError()), "");
+    return rc;
+}
+
 extern "C" int swehr_snapshot_bytes(swehr_handle *h, int64_t *bytes)
 {
     if (!h || !bytes) return SWEHR_ERR_BAD_INPUT;
-    *bytes = (int64_t)h->bytes;
+    *bytes = (int64_t)(kSnapHeader + h->bytes);
     return SWEHR_OK;
 }
 
@@ -788,7 +840,10 @@ extern "C" int swehr_snapshot(swehr_handle *h, void *host)
 {
     if (!h || !host) return SWEHR_ERR_BAD_INPUT;
     HIPCHK(hipSetDevice(h->device));
-    HIPCHK(hipMemcpyAsync(host, h->mem, h->bytes, hipMemcpyDeviceToHost, h->stream));
+    const SnapHeader hd = snap_header(h);
+    memset(host, 0, kSnapHeader);
+    memcpy(host, &hd, sizeof hd);
+    HIPCHK(hipMemcpyAsync((char *)host + kSnapHeader, h->mem, h->bytes, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return SWEHR_OK;
 }
@@ -796,8 +851,27 @@ extern "C" int swehr_snapshot(swehr_handle *h, void *host)
 extern "C" int swehr_restore(swehr_handle *h, const void *host)
 {
     if (!h || !host) return SWEHR_ERR_BAD_INPUT;
+    SnapHeader got;
+    memcpy(&got, host, sizeof got);
+    const SnapHeader want = snap_header(h);
+    if (got.magic != kSnapMagic) {
+        swehr_set_error("restore: not a snapshot of this library%s%s", "", "");
+        return SWEHR_ERR_BAD_INPUT;
+    }
+    if (got.bytes != want.bytes || got.nx != want.nx || got.ny != want.ny || got.S != want.S ||
+        got.precision != want.precision || got.ks_raster != want.ks_raster || got.stage_cap != want.stage_cap ||
+        got.bud_slots != want.bud_slots) {
+        swehr_set_error("restore: the snapshot was taken from an instance of another layout "
+                        "(grid, classes, precision, Ks raster or run length)%s%s", "", "");
+        return SWEHR_ERR_BAD_INPUT;
+    }
+    if (got.rasters != want.rasters) {  // the constant parameter rasters are not in a snapshot
+        swehr_set_error("restore: set the same parameter rasters (hf, dtheta, roughness) on the instance "
+                        "before restoring%s%s", "", "");
+        return SWEHR_ERR_BAD_INPUT;
+    }
     HIPCHK(hipSetDevice(h->device));
-    HIPCHK(hipMemcpyAsync(h->mem, host, h->bytes, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->mem, (const char *)host + kSnapHeader, h->bytes, hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return SWEHR_OK;
 }

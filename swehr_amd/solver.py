@@ -71,6 +71,7 @@ def lib():
     L.swehr_tile_shape.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.swehr_get_stream.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
     L.swehr_profile.argtypes = [C.c_void_p, C.c_int, P, C.POINTER(C.c_int)]
+    L.swehr_profile_window.argtypes = [C.c_void_p, C.c_int, P, C.POINTER(C.c_int)]
     L.swehr_snapshot_bytes.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
     L.swehr_snapshot.argtypes = [C.c_void_p, C.c_void_p]
     L.swehr_restore.argtypes = [C.c_void_p, C.c_void_p]
@@ -242,6 +243,14 @@ class Solver:
         check("swehr_profile", lib().swehr_profile(self._h, int(nsteps), dptr(out), C.byref(n)))
         return out, n.value
 
+    def profile_window(self, nsteps: int):
+        """As profile(), then the state is put back on the device: the profiled steps
+        are the ones the next advance() takes.  Returns (avg_ms[3], steps_profiled)."""
+        out = np.zeros(3)
+        n = C.c_int(0)
+        check("swehr_profile_window", lib().swehr_profile_window(self._h, int(nsteps), dptr(out), C.byref(n)))
+        return out, n.value
+
     def stream(self) -> int:
         s = C.c_void_p()
         check("swehr_get_stream", lib().swehr_get_stream(self._h, C.byref(s)))
@@ -255,7 +264,13 @@ class Solver:
         return buf
 
     def restore(self, buf: np.ndarray):
-        buf = np.ascontiguousarray(buf, dtype=np.uint8)
+        """Load a snapshot of an instance with the same layout (the C side checks the
+        header; the size is checked here, where the buffer's length is known)."""
+        buf = np.ascontiguousarray(buf, dtype=np.uint8).reshape(-1)
+        n = C.c_int64(0)
+        check("swehr_snapshot_bytes", lib().swehr_snapshot_bytes(self._h, C.byref(n)))
+        if buf.nbytes != n.value:
+            raise ValueError("snapshot of %d bytes, this instance's snapshots hold %d" % (buf.nbytes, n.value))
         check("swehr_restore", lib().swehr_restore(self._h, buf.ctypes.data_as(C.c_void_p)))
 
     def autotune(self, nsteps: int = 10):

@@ -15,17 +15,20 @@ from swehr_amd._abi import dptr
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+#: the same oracle source compiled with FMA contraction: a second FP64 evaluation of the
+#: same arithmetic that differs only in the rounding of a*b+c expressions
+FMA_LIB_PATH = os.path.join(ROOT, "oracle", "liboracle_fma.so")
 RUN_PATH = os.path.join(ROOT, "oracle", "_bin", "swehr_oracle_run")
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
-        L = C.CDLL(LIB_PATH)
+def lib(variant: str = "plain"):
+    if variant not in _libs:
+        path = LIB_PATH if variant == "plain" else FMA_LIB_PATH
+        if not os.path.exists(path):
+            raise RuntimeError("%s is not built (run __graft_entry__.build())" % os.path.relpath(path, ROOT))
+        L = C.CDLL(path)
         L.oracle_create.restype = C.c_void_p
         L.oracle_create.argtypes = [C.POINTER(_abi.Params)] + [C.POINTER(C.c_double)] * 5
         L.oracle_step.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
@@ -42,15 +45,17 @@ def lib():
         L.oracle_fail_events.argtypes = [C.c_void_p]
         L.oracle_set_threads.argtypes = [C.c_int]
         L.oracle_destroy.argtypes = [C.c_void_p]
-        _lib = L
-    return _lib
+        _libs[variant] = L
+    return _libs[variant]
 
 
 class Oracle:
     """One oracle instance for a swehr_amd.inputs.Case (FP64)."""
 
-    def __init__(self, case, threads: int = 1):
-        L = lib()
+    def __init__(self, case, threads: int = 1, variant: str = "plain"):
+        """variant "plain" (the checker) or "fma" (the rounding floor: FMA contraction)."""
+        L = lib(variant)
+        self._L = L
         L.oracle_set_threads(int(threads))
         self.case = case
         self._p = _abi.params_from_case(case)
@@ -70,19 +75,19 @@ class Oracle:
 
     def step(self) -> int:
         pd = C.c_int(0)
-        return lib().oracle_step(self._h, C.byref(pd))
+        return self._L.oracle_step(self._h, C.byref(pd))
 
     def run(self, n: int = 1 << 62) -> int:
-        return lib().oracle_run(self._h, int(n))
+        return self._L.oracle_run(self._h, int(n))
 
     def info(self) -> dict:
         i = _abi.StepInfo()
-        lib().oracle_info(self._h, C.byref(i))
+        self._L.oracle_info(self._h, C.byref(i))
         return i.as_dict()
 
     def field(self, name: str, k: int = 0) -> np.ndarray:
         out = np.empty(self.case.nx * self.case.ny, dtype=np.float64)
-        rc = lib().oracle_download(self._h, _abi.FIELDS[name], int(k), dptr(out))
+        rc = self._L.oracle_download(self._h, _abi.FIELDS[name], int(k), dptr(out))
         if rc != 0:
             raise ValueError("oracle_download(%s, %d) -> %d" % (name, k, rc))
         return out.reshape(self.case.nx, self.case.ny)
@@ -92,7 +97,7 @@ class Oracle:
         st = np.zeros(_abi.STAGE_COLS * cap)
         fl = np.zeros(_abi.FLOW_COLS * cap)
         cnt = C.c_int(0)
-        lib().oracle_stage(self._h, dptr(st), dptr(fl), cap, C.byref(cnt))
+        self._L.oracle_stage(self._h, dptr(st), dptr(fl), cap, C.byref(cnt))
         n = cnt.value
         return st.reshape(_abi.STAGE_COLS, cap)[:, :n].copy(), fl.reshape(_abi.FLOW_COLS, cap)[:, :n].copy()
 
@@ -101,16 +106,16 @@ class Oracle:
         cap = _abi.STAGE_MAX
         q, qs = np.zeros(cap), np.zeros(cap)
         cnt = C.c_int(0)
-        lib().oracle_discharge(self._h, dptr(q), dptr(qs), cap, C.byref(cnt))
+        self._L.oracle_discharge(self._h, dptr(q), dptr(qs), cap, C.byref(cnt))
         return q[:cnt.value].copy(), qs[:cnt.value].copy()
 
     def budget(self) -> np.ndarray:
         out = np.zeros(_abi.BUDGET_N)
-        lib().oracle_budget(self._h, dptr(out))
+        self._L.oracle_budget(self._h, dptr(out))
         return out
 
     def fail_events(self) -> int:
-        return int(lib().oracle_fail_events(self._h))
+        return int(self._L.oracle_fail_events(self._h))
 
     def state(self) -> dict:
         """All persistent fields as (nx, ny) arrays; per-class fields as (S, nx, ny)."""
@@ -123,7 +128,7 @@ class Oracle:
 
     def close(self):
         if self._h:
-            lib().oracle_destroy(self._h)
+            self._L.oracle_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -69,3 +69,76 @@ def many_classes_case(S: int, nx: int, ny: int, seed: int):
                 ds=log_spaced_classes(S), pfrac=rng.dirichlet(np.full(S, 2.0)), a=2000.0, ad=4000.0, F=0.01,
                 sidecar=dict(cstable=0.5, ks=2e-6, hf=0.1, theta0=0.1, thetas=0.4, roughness=0.03), movies=0,
                 name="classes%d_%dx%d" % (S, nx, ny))
+
+
+def channel_case():
+    """The inputs of the channel-network configuration at 72 x 56 (ragged against the
+    tiles): fractal surface with carved channels, a per-cell Ks raster, 10 classes with
+    Dirichlet fractions and a rain table of 10 s entries falling from the hyetograph's
+    peak, so that the surface runs off and then recedes within a few hundred steps."""
+    from swehr_amd.inputs import carve_channels, config_c3, fractal_topography
+
+    case = config_c3(seed=21, n=64)
+    rng = np.random.default_rng(21)
+    case.nx, case.ny = 72, 56
+    case.topo = carve_channels(fractal_topography(72, 56, 1.0, 20.0, 0.05, 0.8, rng), 1.0, rng)
+    case.ks_raster = 2.7778e-6 * np.exp(0.5 * rng.standard_normal((72, 56)))
+    case.rain = float(case.rain.max()) * (1.0 - np.arange(len(case.rain)) / len(case.rain))
+    return case
+
+
+def perturbed_ks(case, seed: int, eps: float = 1e-15):
+    """The case with Ks multiplied by (1 + eps N(0,1)) per cell (a uniform Ks becomes a
+    raster of its value): a change of the inputs far below what the physics resolves."""
+    import copy
+
+    p = copy.copy(case)
+    rng = np.random.default_rng(seed)
+    base = case.ks_raster if case.ks_raster is not None else np.full((case.nx, case.ny), case.side("ks"))
+    p.ks_raster = base * (1 + eps * rng.standard_normal(base.shape))
+    return p
+
+
+def rounding_floor(case, steps, ref_state, threads: int = 4):
+    """Per field, how far FP64 evaluations of the same model that differ only in
+    rounding land from the oracle after `steps` steps (None: the whole run): the larger
+    rel L2 of (a) the oracle compiled with FMA contraction, which rounds every a*b+c it
+    contracts once instead of twice, and (b) the oracle with Ks x (1 + 1e-15 N(0,1)).
+    Fields whose per-step increments are differences of nearly equal numbers (the
+    finest classes' deposition, u - chtemp with exp(-vf dt / 2h) within 1e-7 of 1) move
+    by ~1e-11 under (a); runs past a threshold flip move by orders more under both."""
+    from oracle_lib import Oracle
+
+    fl = {}
+    for o in (Oracle(case, threads=threads, variant="fma"), Oracle(perturbed_ks(case, 1), threads=threads)):
+        o.run(steps if steps is not None else 1 << 62)
+        for k, e in compare_states(o.state(), ref_state, case.nclass).items():
+            fl[k] = max(fl.get(k, 0.0), e)
+        o.close()
+    return fl
+
+
+#: a rounding floor above this on some field means the run is past a threshold flip
+CHAOTIC_FLOOR = 1e-6
+
+
+#: fields derived from the state as ratios over the depth (SWEHR.c:1516-1531, 1587):
+#: near-dry cells, where h is a few film depths, dominate their L2 norms
+DERIVED = ("totsed", "velx", "vely", "maxvel")
+
+
+def fp64_violations(errs: dict, floor: dict, rtol: float, k: float):
+    """Fields whose GPU error exceeds what rounding explains.  While the floor is at
+    rounding level (every field below CHAOTIC_FLOOR), per field: max(rtol, k x the
+    floor of that field).  Past a threshold flip, which fields a flip moves most
+    differs from one perturbation to the next (one cell's deposited classes, or its
+    momentum), so the worst prognostic field (DERIVED left out: one near-dry cell's
+    concentration or velocity can make their whole norm) is compared with the floor's
+    worst prognostic field."""
+    fmax = max(floor.values()) if floor else 0.0
+    if fmax > CHAOTIC_FLOOR:
+        prog = [f for f in errs if f not in DERIVED]
+        kw = max(prog, key=lambda f: errs[f])
+        fw = max(floor.get(f, 0.0) for f in prog)
+        return {} if errs[kw] <= k * fw else {kw: (errs[kw], fw)}
+    return {f: (e, floor.get(f, 0.0)) for f, e in errs.items() if not e <= max(rtol, k * floor.get(f, 0.0))}

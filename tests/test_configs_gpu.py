@@ -1,0 +1,374 @@
+"""Each benchmark configuration's workload on the HIP path against the CPU oracle.
+
+The configurations (inputs generated by swehr_amd.inputs):
+  C1  100 x 40 plane, dx 0.5 m, 3 classes, 1200 s: the reference's CPU-runnable case,
+      run whole on the GPU in FP64 (and FP32) against the oracle's whole run;
+  C2  the 2048^2 fractal surface with 7 classes and a Ks raster: the same generator at
+      96^2 run to t = 600 s in FP64 against the oracle, and the full 2048^2 grid for
+      2400 FP64 steps (to t = 600 s) with its water and sediment budgets closed;
+  C4  ensemble hillslopes (per-instance size, dx, slope, roughness, rain, Ks, dtheta,
+      5 classes): small members stepped together in one ensemble for 600 s, FP64
+      against the oracle and FP32 against the FP64 oracle;
+  C5  serving requests (3-10 classes, triangular hyetographs, 600-900 s): small
+      requests through the continuous-batching server, FP64 and FP32, against the
+      oracle.
+
+Bounds.  The FP64 kernel replaces libm's exp, pow and the cos(atan) identities by its
+own evaluations (each within about an ulp) and divisions by refined reciprocals, so it
+matches the oracle to rounding level, which is not one number for every field: where
+an increment of the step is a difference of nearly equal numbers, one ulp becomes a
+large relative change (the finest class's deposition u - chtemp, with the settling
+decay exp(-vf dt / 2h) within 1e-7 of 1: an ulp of the decay is 1e-9 of the
+deposition), and past a cell receding through the film-depth thresholds
+(SWEHR.c:1466-1471, 1532-1551) at a last-bit difference the runs differ by far more.
+So every FP64 field must be within max(RTOL_FP64, K_FLOOR x its rounding floor), the
+floor being how far FP64 evaluations of the same model that differ only in rounding
+land from the oracle (parity_util.rounding_floor: the oracle compiled with FMA
+contraction, and the oracle with Ks perturbed by 1e-15); past a threshold flip the
+worst field is compared with the floor's worst (parity_util.fp64_violations).  Long
+runs are compared at the same simulated time (tend), where the step counts may differ
+by a few steps.
+"""
+import numpy as np
+import pytest
+
+from oracle_lib import Oracle
+from parity_util import channel_case, compare_states, fp64_violations, rel_l2, rounding_floor, worst
+from swehr_amd import _abi
+from swehr_amd import solver as sv
+from swehr_amd.inputs import config_c1, config_c2, config_c4_member, config_c5_request
+
+pytestmark = pytest.mark.gpu
+
+#: FP64 GPU vs FP64 oracle on fields whose rounding floor is below it
+RTOL_FP64 = 1e-12
+#: GPU distance from the oracle over the rounding floor, per field
+K_FLOOR = 4.0
+#: FP32 form against the FP64 oracle (depth rel L2), the target of the precision form
+RTOL_FP32_DEPTH = 1e-4
+
+
+def _report(tag, errs):
+    k, e = worst(errs)
+    print("%s: worst %s %.3e | depth %.3e ch1 %.3e m1 %.3e" % (tag, k, e, errs["depth"], errs.get("ch1", 0),
+                                                                errs.get("m1", 0)), flush=True)
+    return k, e
+
+
+def _assert_fp64(tag, case, errs, steps, ref_state):
+    """Every field within RTOL_FP64, or within K_FLOOR times its rounding floor."""
+    k, e = _report(tag, errs)
+    if e <= RTOL_FP64:
+        return
+    fl = rounding_floor(case, steps, ref_state)
+    kf = max(fl, key=lambda x: fl[x])
+    print("   rounding floor: worst %s %.3e; %s GPU %.3e floor %.3e" % (kf, fl[kf], k, e, fl.get(k, 0.0)), flush=True)
+    bad = fp64_violations(errs, fl, RTOL_FP64, K_FLOOR)
+    assert not bad, (tag, bad)
+
+
+def _assert_fp32_depth(tag, case, e, steps, ref_state):
+    """FP32 depth within RTOL_FP32_DEPTH of the FP64 oracle, or within K_FLOOR times the
+    FP64 rounding floor of the depth where the run meets a threshold flip (FP64 itself is
+    that far from the oracle there)."""
+    if e <= RTOL_FP32_DEPTH:
+        return
+    fl = rounding_floor(case, steps, ref_state)["depth"]
+    print("   %s: FP64 depth rounding floor %.3e" % (tag, fl), flush=True)
+    assert e <= K_FLOOR * fl, (tag, e, fl)
+
+
+def _budget_rel(b):
+    wscale = max(abs(b[n]) for n in (0, 1, 2, 3, 9, 10))
+    sscale = max(max(abs(b[n]) for n in (4, 5, 7, 8)), 1e-30)
+    return abs(sv.water_residual(b)) / wscale, abs(sv.sediment_residual(b)) / sscale
+
+
+def _steps_close(a, b, rel=1e-3):
+    """Step counts of two runs to the same tend: FP64 runs past a flip differ by a few
+    steps (dt follows the fastest wave), FP32 runs by up to 1 %."""
+    return abs(a - b) <= max(2, int(b * rel))
+
+
+# ----------------------------------------------------------------- C1
+
+def test_c1_plane_whole_run_fp64_matches_oracle():
+    """C1 in full (1200 s, 4800 steps at courant 0.5): every field, the outlet stage
+    series and the outlet hydrograph/sedigraph, FP64 GPU against the FP64 oracle
+    (measured: depth bit-equal, every field within 1.1e-15)."""
+    case = config_c1()
+    print("C1 plane %dx%d S=%d whole run, FP64" % (case.nx, case.ny, case.nclass), flush=True)
+    o = Oracle(case, threads=4)
+    o.run()
+    with sv.Solver(case) as s:
+        s.run()
+        si, oi = s.info(), o.info()
+        assert si["done"] == oi["done"] == 1
+        assert si["steps"] == oi["steps"] == 4800
+        assert si["t"] == pytest.approx(oi["t"], rel=1e-14)
+        errs = compare_states(s.state(), o.state(), case.nclass)
+        _, e = _report("C1 fp64", errs)
+        assert e <= RTOL_FP64
+        st, fl = s.stage()
+        so, fo = o.stage()
+        assert st.shape == so.shape
+        assert rel_l2(st, so) <= RTOL_FP64 and rel_l2(fl, fo) <= RTOL_FP64
+        q, qs = s.discharge()
+        oq, oqs = o.discharge()
+        print("C1 hydrograph %.3e sedigraph %.3e" % (rel_l2(q, oq), rel_l2(qs, oqs)))
+        assert rel_l2(q, oq) <= RTOL_FP64 and rel_l2(qs, oqs) <= RTOL_FP64
+    o.close()
+
+
+def test_c1_plane_whole_run_fp32_close_to_fp64_oracle():
+    """C1 in the precision form: depth, velocity and every class's concentration against
+    the FP64 oracle (rel L2), the outlet hydrograph and sedigraph, and the mass budgets
+    (measured: depth 2.5e-8, the float spacing of the film depth; hydrograph 7.6e-8,
+    sedigraph 1.4e-7; budgets 1.4e-8 and 9.1e-9)."""
+    case = config_c1()
+    print("C1 plane whole run, FP32", flush=True)
+    o = Oracle(case, threads=4)
+    o.run()
+    with sv.Solver(case, precision=_abi.FP32) as s:
+        s.run()
+        assert s.info()["steps"] == o.info()["steps"]
+        errs = {"depth": rel_l2(s.field("depth"), o.field("depth")),
+                "speed": rel_l2(s.field("speed"), o.field("speed"))}
+        for k in range(case.nclass):
+            errs["sed%d" % (k + 1)] = rel_l2(s.field("sed", k + 1), o.field("sed", k + 1))
+        q, qs = s.discharge()
+        oq, oqs = o.discharge()
+        errs["hydrograph"] = rel_l2(q, oq)
+        errs["sedigraph"] = rel_l2(qs, oqs)
+        wr, sr = _budget_rel(s.budget())
+    o.close()
+    print("C1 fp32:", {k: "%.2e" % v for k, v in errs.items()}, "budgets %.2e %.2e" % (wr, sr), flush=True)
+    assert errs["depth"] <= 1e-6 and errs["speed"] <= 1e-5
+    assert max(errs["sed%d" % (k + 1)] for k in range(case.nclass)) <= 1e-5
+    assert errs["hydrograph"] <= 1e-6 and errs["sedigraph"] <= 1e-5
+    assert wr <= 1e-6 and sr <= 1e-6
+
+
+# ----------------------------------------------------------------- C2
+
+def test_c2_fractal_fp64_matches_oracle():
+    """The C2 generator (fractal surface on a 5 % tilt, 7 classes 2 um - 2 mm, 80 mm/h,
+    log-normal Ks raster, courant 0.5) at 96^2, run to t = 600 s (runoff established)."""
+    case = config_c2(seed=2, n=96, tend=600.0)
+    print("C2 fractal 96^2 S=7, FP64 to t = 600 s", flush=True)
+    o = Oracle(case, threads=4)
+    o.run()
+    with sv.Solver(case) as s:
+        s.run()
+        si, oi = s.info(), o.info()
+        assert si["done"] == oi["done"] == 1 and _steps_close(si["steps"], oi["steps"])
+        assert float(o.field("depth").max()) > 1e-3  # running off, not just wetting
+        errs = compare_states(s.state(), o.state(), case.nclass)
+        print("C2 96^2: %d / %d steps" % (si["steps"], oi["steps"]))
+        _assert_fp64("C2 96^2 fp64", case, errs, None, o.state())
+    o.close()
+
+
+def test_c2_full_grid_fp64_budgets_close():
+    """The C2 workload at its full 2048^2 size for 2400 FP64 steps (to t = 600 s): every
+    field finite and the water and sediment budgets closed to 1e-12 of the largest
+    term, as an exact FP64 run must (measured: 4e-15 and 0)."""
+    case = config_c2(seed=2, n=2048)
+    print("C2 2048^2 S=7, 2400 FP64 steps", flush=True)
+    with sv.Solver(case) as s:
+        info = s.run(2400)
+        assert info["steps"] == 2400 and info["status"] == 0
+        assert 590.0 <= info["t"] <= 600.0 + 1e-9
+        for f in ("depth", "uh", "vh", "topo", "vinf", "totm"):
+            assert np.all(np.isfinite(s.field(f))), f
+        for k in range(case.nclass):
+            assert np.all(np.isfinite(s.field("ch", k + 1)))
+        b = s.budget()
+    wr, sr = _budget_rel(b)
+    print("C2 2048^2 budget residuals %.3e %.3e" % (wr, sr))
+    assert b[0] > 0 and b[1] > 0 and b[3] > 0  # ponding, infiltrating, running off
+    assert wr <= 1e-12 and sr <= 1e-12
+
+
+# ----------------------------------------------------------------- C4
+
+def _c4_members(n=5, seed=4):
+    rng = np.random.default_rng(seed)
+    return [config_c4_member(rng, k, tend=600.0, scale_max=2.0) for k in range(n)]
+
+
+def test_c4_ensemble_members_fp64_match_oracle():
+    """Five C4 hillslopes (sizes 64x32 - 128x64, dx, slope, roughness, rain, Ks and
+    dtheta drawn per member) stepped together in one ensemble for 600 s each (their
+    own dt), each against its own oracle run."""
+    cases = _c4_members()
+    print("C4 members", [(c.nx, c.ny, round(c.dx, 3)) for c in cases], flush=True)
+    with sv.Batch(cases, precision=_abi.FP64) as b:
+        assert b.advance() == 0
+        for i, c in enumerate(cases):
+            o = Oracle(c, threads=4)
+            o.run()
+            m = b.member(i)
+            assert _steps_close(m.info()["steps"], o.info()["steps"]), c.name
+            errs = compare_states(m.state(), o.state(), c.nclass)
+            _assert_fp64("C4 %s fp64 (%d steps)" % (c.name, o.info()["steps"]), c, errs, None, o.state())
+            o.close()
+
+
+def test_c4_ensemble_members_fp32_close_to_fp64_oracle():
+    """The same members in the FP32 form, against the FP64 oracle: depth rel L2 per
+    member, and the members' water and sediment budgets within 1e-6."""
+    cases = _c4_members()
+    with sv.Batch(cases, precision=_abi.FP32) as b:
+        assert b.advance() == 0
+        for i, c in enumerate(cases):
+            o = Oracle(c, threads=4)
+            o.run()
+            m = b.member(i)
+            e = rel_l2(m.field("depth"), o.field("depth"))
+            es = max(rel_l2(m.field("sed", k + 1), o.field("sed", k + 1)) for k in range(c.nclass))
+            wr, sr = _budget_rel(m.budget())
+            print("C4 %s fp32: depth %.3e sed %.3e budgets %.2e %.2e" % (c.name, e, es, wr, sr), flush=True)
+            assert wr <= 1e-6 and sr <= 1e-6
+            _assert_fp32_depth(c.name, c, e, None, o.state())
+            o.close()
+
+
+# ----------------------------------------------------------------- C5
+
+def _c5_requests(n=3, seed=51):
+    rng = np.random.default_rng(seed)
+    return [config_c5_request(rng, k, sizes=(32, 48, 64), weights=(1, 1, 1), tmin=600.0, tmax=900.0)
+            for k in range(n)]
+
+
+@pytest.mark.parametrize("precision", [_abi.FP64, _abi.FP32], ids=["fp64", "fp32"])
+def test_c5_served_requests_match_oracle(precision):
+    """C5 requests (3-10 classes, Dirichlet fractions, triangular hyetographs, Ks
+    raster, 600-900 s) through the continuous-batching server with room for two at a
+    time, each result against the oracle's run of the same request."""
+    reqs = _c5_requests()
+    print("C5 requests", [(c.nx, c.nclass, round(c.tend, 1)) for c in reqs], "precision", precision, flush=True)
+    budget = int(2.2 * max(sv.instance_bytes_for(c, precision) for c in reqs))
+    done = 0
+    with sv.Server(mem_bytes=budget, max_slots=2, precision=precision) as srv:
+        ids = {srv.submit(c): c for c in reqs}
+        for _ in range(20000):
+            srv.pump(16)
+            while True:
+                d = srv.next_done()
+                if d is None:
+                    break
+                rid, info = d
+                c = ids[rid]
+                o = Oracle(c, threads=4)
+                o.run()
+                assert _steps_close(info["steps"], o.info()["steps"], 1e-3 if precision == _abi.FP64 else 1e-2), \
+                    (c.name, info["steps"], o.info()["steps"])
+                r = srv.result(rid)
+                if precision == _abi.FP64:
+                    errs = compare_states(r.state(), o.state(), c.nclass)
+                    _assert_fp64("C5 %s fp64" % c.name, c, errs, None, o.state())
+                else:
+                    e = rel_l2(r.field("depth"), o.field("depth"))
+                    print("C5 %s fp32 depth %.3e" % (c.name, e), flush=True)
+                    _assert_fp32_depth(c.name, c, e, None, o.state())
+                o.close()
+                srv.release(rid)
+                done += 1
+            if done == len(reqs):
+                break
+    assert done == len(reqs)
+
+
+# ----------------------------------------------------------------- long runs
+
+@pytest.mark.parametrize("steps", [1000, 1600])
+def test_channel_network_long_run_within_rounding_floor(steps):
+    """The channel-network inputs (carved fractal surface, Ks raster, 10 classes, 10 s
+    rain table) past the point where a receding cell meets the film-depth threshold at a
+    last-bit difference (~340 steps; the oracle itself then moves by 2e-3 under a 1e-15
+    change of Ks, test_oracle_invariants): the FP64 GPU's distance from the oracle, per
+    field, within K_FLOOR times the rounding floor."""
+    case = channel_case()
+    print("channel case 72x56 S=10, %d FP64 steps" % steps, flush=True)
+    o = Oracle(case, threads=4)
+    o.run(steps)
+    ref = o.state()
+    with sv.Solver(case) as s:
+        s.run(steps)
+        assert s.info()["steps"] == steps
+        errs = compare_states(s.state(), ref, case.nclass)
+    k, e = _report("channel %d fp64" % steps, errs)
+    fl = rounding_floor(case, steps, ref)
+    print("   rounding floor %.3e (%s %.3e), GPU / floor on %s %.2f"
+          % (max(fl.values()), k, fl[k], k, e / fl[k] if fl[k] else float("inf")))
+    assert e > RTOL_FP64  # the run is past the flip: the bound below is the floor's
+    bad = fp64_violations(errs, fl, RTOL_FP64, K_FLOOR)
+    assert not bad, bad
+    o.close()
+
+
+# ----------------------------------------------------------------- checkpoints
+
+def test_restored_snapshot_continues_like_the_oracle():
+    """A snapshot taken mid-run and restored into a NEW instance continues to the same
+    state as the oracle run straight through (FP64, 400 steps: 150 before the
+    snapshot, 250 after the restore; measured 2.2e-15)."""
+    case = config_c2(seed=7, n=64)
+    o = Oracle(case, threads=4)
+    o.run(400)
+    with sv.Solver(case) as a:
+        a.run(150)
+        snap = a.snapshot()
+        a.run(50)  # the source instance moves on; the snapshot holds step 150
+    with sv.Solver(case) as b:
+        b.restore(snap)
+        assert b.info()["steps"] == 150
+        b.run(250)
+        assert b.info()["steps"] == 400
+        errs = compare_states(b.state(), o.state(), case.nclass)
+    _assert_fp64("restored snapshot", case, errs, 400, o.state())
+    o.close()
+
+
+def test_restore_refuses_a_snapshot_of_another_layout():
+    a_case = config_c2(seed=7, n=64)
+    b_case = config_c2(seed=7, n=48)
+    with sv.Solver(a_case) as a, sv.Solver(b_case) as b, sv.Solver(a_case, precision=_abi.FP32) as c:
+        snap = a.snapshot()
+        with pytest.raises(ValueError):
+            b.restore(snap)  # size differs: refused before the C call
+        with pytest.raises(sv.SwehrError) as ei:  # the C side reads the header
+            sv.check("swehr_restore", sv.lib().swehr_restore(c._h, snap.ctypes.data_as(sv.C.c_void_p)))
+        assert ei.value.code == _abi.ERR_BAD_INPUT
+        snap[:8] = 0  # not a snapshot
+        with pytest.raises(sv.SwehrError):
+            a.restore(snap)
+
+
+def test_evicted_request_finishes_like_the_oracle():
+    """A served FP64 request evicted to host memory mid-run and re-admitted ends at the
+    oracle's state of the same request run straight through."""
+    rng = np.random.default_rng(52)
+    c = config_c5_request(rng, 0, sizes=(48,), weights=(1,), tmin=60.0, tmax=80.0)
+    o = Oracle(c, threads=4)
+    o.run()
+    with sv.Server(mem_bytes=1 << 30, max_slots=2, precision=_abi.FP64) as srv:
+        rid = srv.submit(c)
+        srv.pump(48)
+        srv.evict(rid)
+        assert srv.evictions() == 1
+        res = None
+        for _ in range(20000):
+            srv.pump(16)
+            d = srv.next_done()
+            if d is not None:
+                res = d
+                break
+        assert res is not None and res[0] == rid
+        assert res[1]["steps"] == o.info()["steps"]
+        errs = compare_states(srv.result(rid).state(), o.state(), c.nclass)
+        srv.release(rid)
+    _assert_fp64("evicted request", c, errs, None, o.state())
+    o.close()

@@ -105,3 +105,27 @@ def test_parameter_raster_files_round_trip(tmp_path):
     r = read_case(str(tmp_path))
     for a in ("hf_raster", "dtheta_raster", "roughness_raster"):
         np.testing.assert_array_equal(getattr(r, a), getattr(c, a))
+
+
+def test_oracle_self_divergence_after_a_threshold_flip():
+    """The long-run bound of the GPU parity tests rests on this property of the FP64
+    model itself: on the channel-network inputs, the oracle and the oracle with Ks
+    multiplied by (1 + 1e-15 N(0,1)) agree to rounding level for 300 steps, and some 40
+    steps later a receding cell meets the film-depth test (SWEHR.c:1466-1471,
+    1532-1551) on different sides in the two runs, after which they differ by orders of
+    magnitude more than rounding."""
+    from parity_util import channel_case, compare_states, perturbed_ks, worst
+
+    case = channel_case()
+    out = {}
+    for steps in (300, 400):
+        a = Oracle(case, threads=4)
+        a.run(steps)
+        b = Oracle(perturbed_ks(case, 1), threads=4)
+        b.run(steps)
+        out[steps] = worst(compare_states(b.state(), a.state(), case.nclass))[1]
+        a.close()
+        b.close()
+    print(out)
+    assert out[300] <= 1e-12
+    assert out[400] >= 1e-6
