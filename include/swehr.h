@@ -190,6 +190,15 @@ int swehr_discharge(swehr_handle *h, double *q, double *qs, int capacity, int *c
  *  [10] water from sediment-volume exchange (m^3)  [11] steps accumulated */
 int swehr_budget(swehr_handle *h, double *out);
 
+/* swehr_budget without any allocation: the same 12 terms, bit for bit, with the
+ * per-block partial sums and their compensated reduction in a fixed device array of the
+ * library (80 KiB per GPU, held until the process ends; at most 1024 partial blocks,
+ * which every instance's budget needs at most) instead of the instance's memory and a
+ * host vector.  No counterpart in the reference; it is for callers that cannot
+ * allocate (tight serving loops, harnesses that forbid allocation).  All calls on one
+ * GPU share that array: calls that overlap on one GPU race. */
+int swehr_budget_noalloc(swehr_handle *h, double *out);
+
 /* Run-loop options.  pause_on_print: 1 (default) pause swehr_advance() at each print
  * interval so the caller can write the reference's progress line and movie frame
  * (SWEHR.c:1679-1698); 0 never pause.  steps_per_graph: time steps captured per HIP

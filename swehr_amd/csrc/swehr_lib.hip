@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -1026,6 +1027,104 @@ extern "C" int swehr_budget(swehr_handle *h, double *out)
     out[8] = s[5 + B_DET] + c[5 + B_DET];
     out[9] = s[5 + B_CLAMP] + c[5 + B_CLAMP];
     out[10] = s[5 + B_EXCH] + c[5 + B_EXCH];
+    out[11] = (double)h->hctrl->steps;
+    return SWEHR_OK;
+}
+
+// ------------------------------------------------------------------ budgets without allocation
+//
+// swehr_budget_noalloc: the budget kernel's per-block partials and their reduction live
+// in `scratch`, a file-scope device array (each GPU holds its own copy until the process
+// ends), instead of the instance's block, and the compensated sum over the blocks runs
+// on the device, so the call allocates nothing on the host or the device.  Every call on
+// one GPU uses that GPU's array: calls that overlap on one GPU (two host threads, two
+// handles) race.
+
+namespace {
+constexpr int kNoallocBlocks = 1024;                    // budget blocks the scratch holds
+constexpr int kNoallocDoubles = kNoallocBlocks * 10 + 16;  // partials, then the 10 totals
+constexpr int kNoallocDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) double scratch[kNoallocDoubles];
+
+namespace swehr {
+// One thread: Neumaier sums over the blocks' partials in block order, the order
+// swehr_budget adds them in on the host, so both entry points give the same bits.
+__global__ void budget_reduce_kernel(const double *part, int blocks, double *tot)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double s[10], c[10];
+    for (int n = 0; n < 10; n++) s[n] = c[n] = 0;
+    for (int b = 0; b < blocks; b++)
+        for (int n = 0; n < 10; n++) neumaier(s[n], c[n], part[(size_t)b * 10 + n]);
+    for (int n = 0; n < 10; n++) tot[n] = s[n] + c[n];
+}
+}  // namespace swehr
+
+namespace {
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kNoallocDevices];  // device address of `scratch` on each GPU
+
+// The device address of this GPU's `scratch`, looked up once per device with that
+// device current.  The array's name in host code is a host-side stand-in, not a device
+// address: kernels and copies take the looked-up pointer.
+int scratch_on(int device, double **ptr)
+{
+    if (device < 0 || device >= kNoallocDevices) {
+        swehr_set_error("GPU index beyond the scratch table of swehr_budget_noalloc (64 devices)%s%s", "", "");
+        return SWEHR_ERR_BAD_INPUT;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            swehr_set_error("hipGetSymbolAddress(scratch): %s%s", hipGetErrorString(e), "");
+            return SWEHR_ERR_HIP;
+        }
+        g_scratch_ptr[device] = (double *)p;
+    }
+    *ptr = g_scratch_ptr[device];
+    return SWEHR_OK;
+}
+}  // namespace
+
+extern "C" int swehr_budget_noalloc(swehr_handle *h, double *out)
+{
+    if (!h || !out) return SWEHR_ERR_BAD_INPUT;
+    if (h->budget_blocks < 1 || h->budget_blocks > kNoallocBlocks) {
+        swehr_set_error("swehr_budget_noalloc: %s budget blocks exceed the scratch's %s", "more", "1024");
+        return SWEHR_ERR_BAD_INPUT;
+    }
+    HIPCHK(hipSetDevice(h->device));
+    double *scr = nullptr;
+    const int rc = scratch_on(h->device, &scr);
+    if (rc) return rc;
+    double *tot = scr + (size_t)kNoallocBlocks * 10;
+    if (h->precision == SWEHR_FP64)
+        hipLaunchKernelGGL(budget_kernel<double>, dim3(h->budget_blocks), dim3(256), 0, h->stream, h->a64, scr);
+    else
+        hipLaunchKernelGGL(budget_kernel<float>, dim3(h->budget_blocks), dim3(256), 0, h->stream, h->a32, scr);
+    hipLaunchKernelGGL(budget_reduce_kernel, dim3(1), dim3(64), 0, h->stream, (const double *)scr, h->budget_blocks,
+                       tot);
+    HIPCHK(hipGetLastError());
+    double t[10];
+    HIPCHK(hipMemcpyAsync(t, tot, sizeof t, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(h->hctrl, h->ctrl, sizeof(DevCtrl), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const double a = h->p.dx * h->p.dx;
+    out[0] = t[0] * a;
+    out[1] = t[1] * a;
+    out[2] = h->hctrl->rain_sum + h->hctrl->rain_comp;
+    out[3] = t[5 + B_BWATER];
+    out[4] = t[2] * a;
+    out[5] = t[3] * a;
+    out[6] = (t[4] - h->z0sum) * a;
+    out[7] = t[5 + B_BSED];
+    out[8] = t[5 + B_DET];
+    out[9] = t[5 + B_CLAMP];
+    out[10] = t[5 + B_EXCH];
     out[11] = (double)h->hctrl->steps;
     return SWEHR_OK;
 }

@@ -55,6 +55,7 @@ def lib():
     L.swehr_discharge.argtypes = [C.c_void_p, P, P, C.c_int, C.POINTER(C.c_int)]
     L.swehr_set_parameter_raster.argtypes = [C.c_void_p, C.c_int, P]
     L.swehr_budget.argtypes = [C.c_void_p, P]
+    L.swehr_budget_noalloc.argtypes = [C.c_void_p, P]
     L.swehr_instance_bytes_for.restype = C.c_int64
     L.swehr_instance_bytes_for.argtypes = [C.POINTER(_abi.Params), C.c_int, C.c_int]
     L.swehr_server_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
