@@ -230,6 +230,12 @@ __device__ __forceinline__ void fold_budget(glob<double> *p)
     p[2] = 0.0;
 }
 
+}  // namespace swehr
+
+#include "swehr_cell.h"
+
+namespace swehr {
+
 // --------------------------------------------------------------------------- K1
 
 // Wave-speed bound of the faces a cell owns: i+1/2 and j+1/2, plus 1/2 at i = 0 or
@@ -433,7 +439,6 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     R *s_sed = s_fy + 3 * NFY;
 
     const R hfilm = R(kHfilm);
-    const R rhos = R(kRhos), rhow = R(kRhow);
     const R inv_rhos = R(1.0 / kRhos);
     const int tid = threadIdx.x;
     if (sizeof(R) == sizeof(double) && tid < 32) s_exp2[tid] = kExp2Frac[tid];
@@ -610,7 +615,6 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     __syncthreads();
 
     const double odxd = A.inv_dx;
-    const R odx = R(odxd);
     const R g = R(kG);
     // GZ of the previous step's flux pass at global cell (gi, gj) (SWEHR.c:1279, read
     // lagged at 1135 and 1203); gi, gj are inside the domain.
@@ -690,289 +694,62 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     if (own || (TWO_CHUNK && S > KS)) {
         const int e = (il + 1) * EJ + (jl + 1);
         const int zi = il + 2, zj = jl + 2;
-        const R h = s_h[e], hu = hu_o, hv = hv_o;
-        const R ts = s_ts[e];
-        const R dt = R(dtd), Rr = R(Rd);
-        const R h0 = R(A.h0);
-
-        // geometry of this step; GX takes SLOPEY of the previous step (SWEHR.c:1239-1280)
-        const R sxn = R(slope_x<ZJ>(s_z, zi, zj, gi, nx, odxd));
-        const R syo = R(slope_y<ZJ>(s_zp, zi, zj, gj, ny, odxd));
-        const R syn = R(slope_y<ZJ>(s_z, zi, zj, gj, ny, odxd));
-        const R gx = g * sxn * mrsqrt(sxn * sxn + syo * syo + R(1));
-        const R sxs = -gx * h;
-        const R n2 = sxn * sxn + syn * syn + R(1);
-        const R rn = mrsqrt(n2);
-        const R gy = g * syn * rn;
-        const R sys = -gy * h;
-        const R gz = g * rn;
-
-        // mixture density from the previous update (SWEHR.c:1578-1579)
-        R rho = rhow * (R(1) - ts) + rhos * ts;
-        if (rho > R(0.8) * rhos) rho = R(0.8) * rhos;
-
-        // variable-density forcing (SWEHR.c:1282-1290); the same factor in x and y
-        R gmx = R(0), gmy = R(0);
-        const bool gx_in = gi > 0 && gi < nx - 1, gy_in = gj > 0 && gj < ny - 1;
-        if (gx_in || gy_in) {
-            const R fac = (rhos - rho) * gz * h * h * mrcp(R(2) * rho) * R(.5) * odx;
-            if (gx_in) gmx -= fac * (s_ts[e + EJ] - s_ts[e - EJ]);
-            if (gy_in) gmy -= fac * (s_ts[e + 1] - s_ts[e - 1]);
-        }
-
-        // stability of the deposited layer (SWEHR.c:1292-1312): right/upper neighbour
-        // slopes of the previous step, left/lower ones of this step, own at the edges;
-        // cos(atan|x|) = 1/msqrt(1+x^2), sin(atan|x|) = |x|/msqrt(1+x^2)
-        const R rho0 = R(kRhow * kPhi + kRhos * (1 - kPhi));
+        // the cell's neighbourhood from the tile's LDS planes; neighbour slopes and DEP
+        // at the domain's edges are the cell's own (SWEHR.c:1292-1312)
         const int du = gi < nx - 1 ? 1 : 0, dd = gi > 0 ? 1 : 0;
         const int ju = gj < ny - 1 ? 1 : 0, jd = gj > 0 ? 1 : 0;
-        const R sx_up = du ? R(slope_x<ZJ>(s_zp, zi + 1, zj, gi + 1, nx, odxd)) : sxn;
-        const R sx_dn = dd ? R(slope_x<ZJ>(s_z, zi - 1, zj, gi - 1, nx, odxd)) : sxn;
-        const R dep = s_dep[e];
-        const R cy = mrsqrt(R(1) + syn * syn);
-        const R load = rho0 * gz * dep + rho * gz * h;
-        const R lhs = load * (mfabs(syn) * cy);
-        const R base = load * cy - rhow * gz * (dep + h);
-        const R kfa = R(kFrictionAngle), kco = R(kCohesion);
-        const R pgf0 = dep * rho0 * gz * R(0.5) * odx;
-        R pgfx = pgf0 * (s_dep[e + du * EJ] * mrsqrt(R(1) + sx_up * sx_up) -
-                         s_dep[e - dd * EJ] * mrsqrt(R(1) + sx_dn * sx_dn));
-        if (pgfx * sxn > R(0)) pgfx = R(-1) * mfabs(pgfx);
-        if (pgfx * sxn < R(0)) pgfx = mfabs(pgfx);
-        bool fail = lhs > (kco + base * kfa + pgfx);
-        if (!fail) {
-            const R sy_up = ju ? R(slope_y<ZJ>(s_zp, zi, zj + 1, gj + 1, ny, odxd)) : syn;
-            const R sy_dn = jd ? R(slope_y<ZJ>(s_z, zi, zj - 1, gj - 1, ny, odxd)) : syn;
-            R pgfy = pgf0 * (s_dep[e + ju] * mrsqrt(R(1) + sy_up * sy_up) - s_dep[e - jd] * mrsqrt(R(1) + sy_dn * sy_dn));
-            if (pgfy * syn > R(0)) pgfy = R(-1) * mfabs(pgfy);
-            if (pgfy * syn < R(0)) pgfy = mfabs(pgfy);
-            fail = lhs > (kco + base * kfa + pgfy);
-        }
-
-        // debris-flow forcing (SWEHR.c:1314-1320)
-        R dfm = mfmin(R(1), (ts - R(0.2)) * R(5));
-        if (h < R(9) * h0 || ts < R(0.2)) dfm = R(0);
-        const R bt = -(R(1) - R(kLambda)) * gz * h * kfa;
-        const R btx = hu < R(0) ? R(-1) * bt : bt;
-        const R bty = hv < R(0) ? R(-1) * bt : bt;
-
-        // class-invariant parts of the Hairsine-Rose sources (SWEHR.c:1329-1378);
-        // depth normal to the bed h / cos(atan|slope|) = h * msqrt(1 + slope^2)
-        const R totm_old = totm_o;
-        const R hvert = h * (n2 * rn);
-        const bool deep_v = hvert > h0;
-        const R ihv = deep_v ? mrcp(hvert) : R(0);
-        const R mtstar = deep_v ? R(A.mtstar0) * (h0 * ihv) : R(A.mtstar0);
-        const R H = mfmin(R(1), totm_old * mrcp(mtstar));  // mtstar = 0: NaN, which fmin drops as inf/NaN
-        const R shield = deep_v ? h0 * ihv : R(1);
-        const R rough = A.n_raster ? R(at(gmem(A.n_raster), c8L)) : R(A.roughness);
-        const bool active = h > R(10) * hfilm && ts < R(0.6);
-        const R oneh = R(1) - mfmin(ts, R(0.99));
-        const R hind = (oneh * oneh) * (oneh * oneh);
-        const bool shutoff = h > R(9) * h0 && ts > R(kCthreshold);
-        const bool shallow_v = h < R(9) * h0;
-        const R vshal = mfmax(h - R(3) * h0, R(0)) * A.inv_6h0;
-        const R cg = R(1.09) * R(kRhos / kRhow - 1) * gz;
-        // rainsplash and stream-power factors; class k multiplies by P_k or M_k
-        R f1 = R(0), f2 = R(0), f3 = R(0), f4 = R(0), omega = R(0);
-        bool e2ok = false, e4ok = false;
-        if (active) {
-            const R qq = hu * hu + hv * hv;
-            const R qm = msqrt(qq);
-            const R sf = manning2_over_hpow<3>(h, rough, s_exp2, s_log) * qq;
-            omega = rho * gz * sf * qm;
-            const R oneH = R(1) - H;
-            f1 = oneH * R(A.a) * Rr * shield;
-            e2ok = totm_old > R(1e-8);
-            e4ok = totm_old > R(1e-6);
-            const R itm = e2ok ? mrcp(totm_old) : R(0);
-            f2 = H * itm * R(A.ad) * Rr * shield;
-            f3 = oneH * R(A.F) / R(A.J) * omega;
-            if (e4ok) f4 = (R(1) - R(kBetaBedload)) * H * itm * R(A.F) * mrcp((rhos - rho) * inv_rhos * gz * h) * omega;
-        }
-        // debris shutoff (SWEHR.c:1391-1399) zeroes every source and the settling: with
-        // the factors zeroed each class's E1..E4 are exactly 0 (E2, E4 are taken only
-        // where M_k is finite and positive), and vf is scaled by 0
-        const R vf_keep = shutoff ? R(0) : R(1);
-        if (shutoff) f1 = f2 = f3 = f4 = R(0);
-
-        // conservative update (SWEHR.c:1424-1441)
+        CellIn<R> ci;
+        ci.h = s_h[e];
+        ci.hu = hu_o;
+        ci.hv = hv_o;
+        ci.ts = s_ts[e];
+        ci.ts_xm = s_ts[e - EJ];
+        ci.ts_xp = s_ts[e + EJ];
+        ci.ts_ym = s_ts[e - 1];
+        ci.ts_yp = s_ts[e + 1];
+        ci.dep = s_dep[e];
+        ci.dep_xu = s_dep[e + du * EJ];
+        ci.dep_xd = s_dep[e - dd * EJ];
+        ci.dep_yu = s_dep[e + ju];
+        ci.dep_yd = s_dep[e - jd];
+        ci.sxn = R(slope_x<ZJ>(s_z, zi, zj, gi, nx, odxd));
+        ci.syo = R(slope_y<ZJ>(s_zp, zi, zj, gj, ny, odxd));
+        ci.syn = R(slope_y<ZJ>(s_z, zi, zj, gj, ny, odxd));
+        ci.sx_up = du ? R(slope_x<ZJ>(s_zp, zi + 1, zj, gi + 1, nx, odxd)) : ci.sxn;
+        ci.sx_dn = dd ? R(slope_x<ZJ>(s_z, zi - 1, zj, gi - 1, nx, odxd)) : ci.sxn;
         const int fxl = il * TJ + jl, fxr = fxl + TJ;
         const int fyl = il * (TJ + 1) + jl, fyr = fyl + 1;
-        const R dx_ = dt * odx;
-        // FP32: the depth's increments of the step (fluxes, rain, infiltration, exchange)
-        // are summed apart in dh_rest and added to the depth once, without bias,
-        // after the class loop.  In deep water they are a few float spacings at h each
-        // step (rain 1.3e-6 m against 9.5e-7 m at 8 m), and rounded to nearest with the
-        // same fraction every step they would gain or lose water systematically.
-        R hn, dh_rest = R(0);
-        bool film = false;  // FP32: the infiltration term clamped the cell to the film depth
-        if constexpr (sizeof(R) == sizeof(double)) {
-            hn = h - dx_ * (s_fx[fxr] - s_fx[fxl]) - dx_ * (s_fy[fyr] - s_fy[fyl]) + dt * Rr;
-        } else {
-            dh_rest = dt * Rr - dx_ * (s_fx[fxr] - s_fx[fxl]) - dx_ * (s_fy[fyr] - s_fy[fyl]);
-            hn = h + dh_rest;
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            ci.fxl[m] = s_fx[m * NFX + fxl];
+            ci.fxr[m] = s_fx[m * NFX + fxr];
+            ci.fyl[m] = s_fy[m * NFY + fyl];
+            ci.fyr[m] = s_fy[m * NFY + fyr];
         }
-        R hun = hu - dx_ * (s_fx[NFX + fxr] - s_fx[NFX + fxl]) - dx_ * (s_fy[NFY + fyr] - s_fy[NFY + fyl]) + dt * sxs +
-                dt * gmx + dt * dfm * btx;
-        R hvn = hv - dx_ * (s_fx[2 * NFX + fxr] - s_fx[2 * NFX + fxl]) -
-                dx_ * (s_fy[2 * NFY + fyr] - s_fy[2 * NFY + fyl]) + dt * sys + dt * gmy + dt * dfm * bty;
+        ci.totm_o = totm_o;
+        ci.mv_o = mv_o;
+        ci.md_o = md_o;
+        ci.vinf_o = vinf_o;
+        ci.ks_o = ks_o;
+        ci.zn = s_z[zi * ZJ + zj];
+        Cell<R> X;
+        X.gi = gi;
+        X.gj = gj;
+        X.own = own;
+        const R syn = ci.syn;
+        cell_begin(X, A, ci, dtd, Rd, steps, c, s_exp2, s_log, c8L, [&](R &up, R &dn) {
+            up = ju ? R(slope_y<ZJ>(s_zp, zi, zj + 1, gj + 1, ny, odxd)) : syn;
+            dn = jd ? R(slope_y<ZJ>(s_z, zi, zj - 1, gj - 1, ny, odxd)) : syn;
+        });
 
-        // Green-Ampt infiltration in FP64; negative below the film depth (SWEHR.c:1448-1452)
-        {
-            const double ks = ks_o;
-            const double vin = vinf_o;
-            const double dth = A.dtheta_raster ? at(gmem(A.dtheta_raster), c8L) : A.thetas - A.theta0;
-            const double hf_c = A.hf_raster ? at(gmem(A.hf_raster), c8L) : A.hf;
-            const double zf = vin * mrcp(dth);
-            const double hd = sizeof(R) == sizeof(double) ? (double)hn : (double)h + (double)dh_rest;
-            // the film depth as stored in this precision, so a dry cell stays exactly dry
-            // zf = 0 gives NaN where division gives inf (or NaN): fmin returns h - hfilm for both
-            const double infl = mfmin(hd - (double)hfilm, dtd * ks * (zf + hf_c + hd) * mrcp(zf));
-            const double vin_new = vin + infl;
-            if (own && !same_bits(vin_new, vin)) st_stream(vinfp, c8, vin_new);
-            hn = R(hd - infl);
-            if constexpr (sizeof(R) == sizeof(float)) {
-                if (infl == hd - (double)hfilm) {  // the film clamp: exactly the film depth
-                    film = true;
-                    dh_rest = R(0);
-                } else {
-                    dh_rest = dh_rest - R(infl);
-                }
-            }
-        }
-
-        // sediment: flux divergence, sources and exchange per class (SWEHR.c:1454-1511)
-        const R fx0l = s_fx[fxl], fx0r = s_fx[fxr], fy0l = s_fy[fyl], fy0r = s_fy[fyr];
         // upwind cell of each face (SWEHR.c:1152-1157, 1219-1224): class-invariant
-        const int uxl = fx0l > R(0) ? e - EJ : e, uxr = fx0r > R(0) ? e : e + EJ;
-        const int uyl = fy0l > R(0) ? e - 1 : e, uyr = fy0r > R(0) ? e : e + 1;
-        const R rsp = rhos * (R(1) - R(kPhi));
-        const R kbed = R(1) / rsp;
-        const R hdt = R(0.5) * dt;
-        double zn_c = s_z[zi * ZJ + zj];
-        R totm = R(0), depn = R(0), chtemp = R(0);
-        bool left_suspended = false;  // some class ends the step with ch_k != 0
-        double sb_out = 0, sq_out = 0;  // sediment leaving through all edges / through j = 1/2
-        // FP32: the exchange with the bed changes h by ~1e-10 m per class and step, far
-        // below the spacing of floats at h (9.3e-10 m at 1 cm) and mostly of one sign, so
-        // added to h one by one it would be lost every step (a systematic water budget
-        // error).  The classes' increments join dh_rest.
-        // random bits of the unbiased roundings (FP32): a hash of the cell and the step,
-        // then a linear congruential sequence (the Numerical Recipes multiplier and
-        // increment, mod 2^32; the high bits are used) over the values the cell stores,
-        // so results are reproducible and do not depend on the tile shape or on the
-        // ensemble
-        unsigned rbits = sizeof(R) == sizeof(float) ? mix32(c ^ mix32((unsigned)steps)) : 0u;
-        auto next_bits = [&]() {
-            rbits = rbits * 1664525u + 1013904223u;
-            return rbits;
-        };
-        const bool edge = gi == 0 || gi == nx - 1 || gj == 0 || gj == ny - 1;
-        // class k from its suspended mass ch_k and deposited mass m_k
+        const int uxl = X.fx0l > R(0) ? e - EJ : e, uxr = X.fx0r > R(0) ? e : e + EJ;
+        const int uyl = X.fy0l > R(0) ? e - 1 : e, uyr = X.fy0r > R(0) ? e : e + 1;
+        // class k from its suspended mass ch_k, deposited mass m_k and the LDS plane sk
+        // of its concentrations
         auto update_class = [&](int k, R ch_k, R m_k, const R *sk) {
-            const R fkxl = fx0l * sk[uxl], fkxr = fx0r * sk[uxr];
-            const R fkyl = fy0l * sk[uyl], fkyr = fy0r * sk[uyr];
-            if (edge) {  // a branch: most waves hold no cell on the domain's edge
-                if (gi == 0) sb_out -= (double)fkxl;
-                if (gi == nx - 1) sb_out += (double)fkxr;
-                if (gj == 0) {
-                    sb_out -= (double)fkyl;
-                    sq_out -= (double)fkyl;
-                }
-                if (gj == ny - 1) sb_out += (double)fkyr;
-            }
-            R u = ch_k - dx_ * (fkxr - fkxl) - dx_ * (fkyr - fkyl);
-            R m = m_k;
-            // sources of class k from the state before this update (SWEHR.c:1322-1399)
-            // (outside active cells f1..f4 and omega are 0 and e2ok, e4ok false, so every
-            // E is 0 as the reference's guard makes it)
-            const R dk = R(A.ds[k]), pk = R(A.pf[k]);
-            const R e1 = pk * f1;
-            const R e2 = (e2ok && m > R(1e-8)) ? m * f2 : R(0);
-            const bool ent = omega > R(0) && h > dk;
-            const R e3 = ent ? pk * f3 : R(0);
-            const R e4 = (ent && e4ok && m > R(0)) ? m * f4 : R(0);
-            // settling velocity sqrt(ak^2 + cg d) - ak (SWEHR.c:435).  For fine classes
-            // cg d is ~1e-6 of ak^2 and the difference keeps only the last digits of the
-            // sqrt: FP64 evaluates it as written, FP32 as cg d / (sqrt(ak^2 + cg d) + ak),
-            // equal in real arithmetic and free of the cancellation
-            const R ak = R(A.ak[k]);
-            R vf;
-            if constexpr (sizeof(R) == sizeof(double)) {
-                vf = msqrt_pos(R(A.ak2[k]) + cg * dk) - ak;
-            } else {
-                const R b = cg * dk;
-                vf = b * mrcp(msqrt_pos(R(A.ak2[k]) + b) + ak);
-            }
-            vf = vf * hind;
-            if (((A.coarse1 >> k) & 1u) && shallow_v) vf = R(2e-3) + vshal * (vf - R(2e-3));
-            if (((A.coarse2 >> k) & 1u) && shallow_v) vf = R(2e-3) + vshal * (vf - R(2e-3));
-            vf = vf * vf_keep;
-            // exchange with the deposited layer
-            // the running depth: in FP32 the depth before the step plus the increments so far
-            const R hcur = sizeof(R) == sizeof(double) ? hn : (film ? hfilm : h) + dh_rest;
-            if (hcur < R(10) * hfilm) {
-                if constexpr (sizeof(R) == sizeof(double)) m += dt * e1 + u;
-                else m = add_unbiased(m, dt * e1 + u, next_bits());
-                bud[B_DET] += (double)(dt * e1);
-                u = R(0);
-            } else {
-                chtemp = u;
-                const R red = mfmin(m, dt * (e2 + e4));
-                if constexpr (sizeof(R) == sizeof(double)) {
-                    const R decay = exp_tab(-vf * hdt * mrcp(hcur), s_exp2);
-                    u = decay * u;
-                    u = u + dt * (e1 + e3) + red;
-                    u = decay * u;
-                    zn_c = zn_c - (double)(kbed * (u - chtemp));
-                    m = m + dt * (e1 + e3) - (u - chtemp);
-                } else {
-                    // FP32: the decay factor d = exp(-vf dt / 2h) of a fine class differs
-                    // from 1 by less than the spacing of floats there (2 um at h = 1 cm:
-                    // 1 - 4.5e-8), so d * u as written rounds to u or overstates the
-                    // settling.  The update d (d u + s) is taken as one increment of u,
-                    // u em1 (2 + em1) + s (1 + em1) with em1 = d - 1 (mexpm1_neg), added
-                    // with one unbiased rounding; M_k takes the difference of the stored
-                    // values and is updated the same way
-                    const R em1 = mexpm1_neg(-vf * hdt * mrcp(hcur));
-                    const R src = dt * (e1 + e3) + red;
-                    const R inc = u * (em1 * (R(2) + em1)) + src * (R(1) + em1);
-                    u = add_unbiased(u, inc, next_bits());
-                    const R dch = u - chtemp;
-                    zn_c = zn_c - (double)(kbed * dch);
-                    m = add_unbiased(m, dt * (e1 + e3) - dch, next_bits());
-                }
-                bud[B_DET] += (double)(dt * (e1 + e3));
-                if (fail) {
-                    if constexpr (sizeof(R) == sizeof(double)) {
-                        zn_c = zn_c - (double)(R(0.99) * m * kbed);
-                        u = u + R(0.99) * m;
-                        m = R(0.01) * m;
-                    } else {
-                        // FP32: the float 0.99 and 0.01 do not sum to 1, so the two
-                        // products would not conserve the class's mass; the mass kept is
-                        // rounded once and the rest moves into suspension
-                        const R keep = R(0.01) * m, moved = m - keep;
-                        zn_c = zn_c - (double)(moved * kbed);
-                        u = add_unbiased(u, moved, next_bits());
-                        m = keep;
-                    }
-                    totm = R(0);
-                    depn = R(0);
-                }
-            }
-            const R dh = (u - chtemp) * kbed;
-            if constexpr (sizeof(R) == sizeof(double)) hn = hn + dh;
-            else dh_rest += dh;
-            bud[B_EXCH] += (double)dh;
-            totm += m;
-            depn += m * inv_rhos;
-            left_suspended = left_suspended || u != R(0);
-            if (own) {
-                st_stream(chout + k * plane, cR, u);
-                if (!same_bits(m, m_k)) st_stream(Mp + k * plane, cR, m);
-            }
+            cell_class(X, A, k, ch_k, m_k, X.fx0l * sk[uxl], X.fx0r * sk[uxr], X.fy0l * sk[uyl], X.fy0r * sk[uyr],
+                       s_exp2, chout, Mp, plane, cR);
         };
         // first chunk, classes 0 .. S0-1.  Below KO: unrolled, ch_k from registers; from
         // KO on ch_k comes from memory, one class ahead.  M_k from registers below MO.
@@ -1067,72 +844,10 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             }
         }
 
-        if constexpr (sizeof(R) == sizeof(float)) hn = add_unbiased(film ? hfilm : h, dh_rest, next_bits());
-
-        // dry-cell clamp (SWEHR.c:1532-1551)
-        if (!(hn > hfilm) && own) {
-            bud[B_CLAMP] += (double)(hfilm - hn);
-            hn = hfilm;
-            hun = R(0);
-            hvn = R(0);
-            // the suspended mass goes to the deposited layer and the bed; where every
-            // class ended with ch_k = 0 (a cell dry through the whole step, the common
-            // case) that changes nothing but the sign of a zero, and the classes are not
-            // read back and rewritten, which would double the cell's traffic
-            glob<R> *chq = chout;
-            R u_a = left_suspended ? at(chq, cR) : R(0), mm_a = left_suspended ? at(Mp, cR) : R(0);
-            for (int k = 0; k < (left_suspended ? S : 0); k++) {
-                const R u = u_a, mm = mm_a;
-                if (k + 1 < S) {
-                    u_a = at(chq + (k + 1) * plane, cR);
-                    mm_a = at(Mp + (k + 1) * plane, cR);
-                }
-                zn_c = zn_c + (double)(kbed * u);
-                // FP32: u is often far below the float spacing at mm
-                if constexpr (sizeof(R) == sizeof(double)) at(Mp + k * plane, cR) = mm + u;
-                else at(Mp + k * plane, cR) = add_unbiased(mm, u, next_bits());
-                at(chq + k * plane, cR) = R(0);
-            }
-        }
-        // implicit friction (SWEHR.c:1553-1565) and maxima (SWEHR.c:1587-1588)
-        R v1 = R(0), v2 = R(0);
-        if (hn > hfilm) {
-            const R sf2 = manning2_over_hpow<2>(hn, rough, s_exp2, s_log) * gz * msqrt(hun * hun + hvn * hvn);
-            const R den = mrcp(R(1) + dt * sf2);
-            hun = hun * den;
-            hvn = hvn * den;
-            const R ihn = mrcp(hn);
-            v1 = hun * ihn;
-            v2 = hvn * ihn;
-        }
-        const R speed = msqrt(v1 * v1 + v2 * v2);
-        if (own) {
-            if (speed > mv_o) st_stream(maxvelp, cR, speed);
-            if (hn > md_o) st_stream(maxdepthp, cR, hn);
-            st_stream(hout, cR, hn);
-            st_stream(huout, cR, hun);
-            st_stream(hvout, cR, hvn);
-            st_stream(zout, c8, zn_c);
-            if (!same_bits(totm, totm_o)) st_stream(totmp, cR, totm);
-            st_stream(depout, cR, depn);
-        }
-
-        // water and sediment leaving through the domain edges this step
-        double wb = 0;
-        if (gi == 0) wb -= (double)fx0l;
-        if (gi == nx - 1) wb += (double)fx0r;
-        if (gj == 0) wb -= (double)fy0l;
-        if (gj == ny - 1) wb += (double)fy0r;
-        bud[B_BWATER] = wb * dtd * A.dx;
-        bud[B_BSED] = sb_out * dtd * A.dx;
-        if (gj == 0 && own) {  // outlet discharge of this column (per unit width times dx)
-            gmem(A.outq)[gi] = -(double)fy0l * A.dx;
-            gmem(A.outs)[gi] = sq_out * A.dx;
-        }
-        const double a2 = A.dx * A.dx;
-        bud[B_EXCH] *= a2;
-        bud[B_CLAMP] *= a2;
-        bud[B_DET] *= a2;
+        cell_end(X, A, dtd, s_exp2, s_log, hout, huout, hvout, chout, depout, zout, Mp, totmp, maxvelp, maxdepthp,
+                 plane, cR, c8);
+#pragma unroll
+        for (int k = 0; k < NB_TERMS; k++) bud[k] = X.bud[k];
     }
 
     // ---- budget partials: each wave stores its sums of this step as the pending
