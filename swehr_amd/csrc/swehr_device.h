@@ -356,36 +356,55 @@ constexpr int kStepWaves = sizeof(R) == 4 ? (SPLIT ? 4 : 5) : 3;
 // 2 on.
 constexpr int kFp32MOwn = 1;
 // FP64 two-chunk variant: classes whose M_k is loaded up front (to 10 classes within
-// its register budget; the first chunk's 7 and three of the second)
+// its register budget, the first chunk's)
 constexpr int kSplitMOwn = 10;
+// FP64: classes whose ch_k and M_k of the own cell stay in registers from the halo
+// staging on (the rest are read one class ahead in the class loop)
+constexpr int kFp64Own = 7;
 // FP32 variant for many classes: classes whose ch_k and M_k of the own cell stay in
 // registers (within the 128-VGPR budget of 4 waves per SIMD)
 constexpr int kFp32WideOwn = 10;
 
 // Size classes whose upwind concentrations have LDS planes of their own (the first
-// chunk); in FP64 the classes above share the planes of the velocities, TOTSED, DEP
-// and face fluxes, which are dead by the time the class loop reaches them.  The FP64
-// tile's LDS is then that of 7 classes for any S, which keeps 3 blocks per CU.  FP32
-// keeps every class in its own planes: its second-chunk variant needs more registers
-// than 5 waves per SIMD allow and measured slower than the LDS-limited single chunk.
+// chunk); in FP64 the classes above share the planes of the velocities, TOTSED, DEP,
+// the exchanged faces and the first chunk, which are dead by the time the class loop
+// reaches them.  An FP64 tile of 8 x 32 with 10 class planes fits 3 blocks per CU.
+// FP32 keeps every class in its own planes: its second-chunk variant needs more
+// registers than 5 waves per SIMD allow and measured slower than the single chunk.
 template <typename R>
-constexpr int kLdsClasses = sizeof(R) == sizeof(double) ? 7 : kMaxClasses;
+constexpr int kLdsClasses = sizeof(R) == sizeof(double) ? 10 : kMaxClasses;
 
 template <typename R, int TI, int TJ>
 struct TileGeom {
     static constexpr int NT = TI * TJ;
     static constexpr int EI = TI + 2, EJ = TJ + 2, EXT = EI * EJ;  // 1-cell halo
     static constexpr int ZI = TI + 4, ZJ = TJ + 4, ZEXT = ZI * ZJ;  // 2-cell halo (bed)
-    static constexpr int NFX = (TI + 1) * TJ, NFY = TI * (TJ + 1);
+    // face fluxes crossing between waves: the rows of a wave (RW = 64 / TJ) pass their
+    // x faces within the wave; LDS holds the x faces at the first row of each wave (NXW
+    // rows of TJ faces: the tile's low edge, then each wave's last row) and the y faces
+    // at the tile's low edge (TI)
+    static constexpr int RW = 64 / TJ, NXW = TI / RW;
+    static constexpr int NFXW = NXW * TJ, NFYE = TI;
     static constexpr int KS = kLdsClasses<R>;
     // the second chunk (classes KS .. S-1) overlays everything from the u plane on
-    static_assert((4 + KS) * EXT + 3 * NFX + 3 * NFY >= (kMaxClasses - KS) * EXT, "second class chunk");
+    static_assert((4 + KS) * EXT + 3 * NFXW + 3 * NFYE >= (kMaxClasses - KS) * EXT, "second class chunk");
+    static_assert(64 % TJ == 0 && TI % RW == 0, "rows of a tile fill whole waves");
     static size_t lds_bytes(int S)
     {
         const int s0 = S < KS ? S : KS;
-        return sizeof(double) * 2 * ZEXT + sizeof(R) * ((size_t)(5 + s0) * EXT + 3 * NFX + 3 * NFY);
+        return sizeof(double) * 2 * ZEXT + sizeof(R) * ((size_t)(5 + s0) * EXT + 3 * NFXW + 3 * NFYE);
     }
 };
+
+// Value held by the lane below (L - 1) by a DPP wave shift; lane 0 keeps its own.
+__device__ __forceinline__ int lane_below_i(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ float lane_below(float v) { return __int_as_float(lane_below_i(__float_as_int(v))); }
+__device__ __forceinline__ double lane_below(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = lane_below_i((int)(unsigned)b), hi = lane_below_i((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
 
 // The fused step for one tile: ghosts, face fluxes, sources, update.
 //
@@ -405,7 +424,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
 {
     using T = TileGeom<R, TI, TJ>;
     constexpr int NT = T::NT, EJ = T::EJ, EXT = T::EXT, ZJ = T::ZJ, ZEXT = T::ZEXT;
-    constexpr int NFX = T::NFX, NFY = T::NFY, KS = T::KS;
+    constexpr int RW = T::RW, NFXW = T::NFXW, NFYE = T::NFYE, KS = T::KS;
     glob<DevCtrl> *C = gmem(A.ctrl);
     if (!step_active(C)) return;
 
@@ -435,8 +454,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     __shared__ double s_exp2[32];  // 2^(j/32) for the FP64 exponentials of phase 3
     __shared__ double s_log[69];   // kLogTab for the FP64 logarithms of phase 3
     R *s_u = s_h + EXT, *s_v = s_u + EXT, *s_ts = s_v + EXT, *s_dep = s_ts + EXT;
-    R *s_fx = s_dep + EXT, *s_fy = s_fx + 3 * NFX;
-    R *s_sed = s_fy + 3 * NFY;
+    R *s_fxw = s_dep + EXT, *s_fye = s_fxw + 3 * NFXW;
+    R *s_sed = s_fye + 3 * NFYE;
 
     const R hfilm = R(kHfilm);
     const R inv_rhos = R(1.0 / kRhos);
@@ -466,7 +485,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     // again there they would come from memory, the halo having left L2 by then
     // (in FP32 only in the SPLIT variant: the register budget of 5 waves per SIMD has
     // no room)
-    constexpr int KO = sizeof(R) == sizeof(double) ? KS : (SPLIT ? kFp32WideOwn : 0);
+    constexpr int KO = sizeof(R) == sizeof(double) ? kFp64Own : (SPLIT ? kFp32WideOwn : 0);
     R ch_own[KO > 0 ? KO : 1];
 #pragma unroll
     for (int k = 0; k < KO; k++) ch_own[k] = k < S ? at(chin + k * plane, cRL) : R(0);
@@ -625,35 +644,30 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         return g * mrsqrt(a * a + bb * bb + R(1));
     };
 
-    // ---- phase 2: face fluxes of the tile, each computed once
-    // x face between ext cells el (left) and el + EJ, with g and cl of the left cell
-    auto xface_g = [&](int fi, int fj, R gz, R cl) {
-        const int el = fi * EJ + fj + 1, er = el + EJ;
+    // ---- phase 2: face fluxes of the tile, each computed once.  A cell's own faces
+    // i+1/2 and j+1/2 (sharing its lagged GZ and celerity) stay in its thread's
+    // registers.  Its j-1/2 face is the j+1/2 face of the lane below, in the same row;
+    // its i-1/2 face the i+1/2 face of the row above, in the same wave except at a
+    // wave's first row.  LDS holds what crosses waves: the i+1/2 faces of each wave's
+    // last row and the faces of the tile's low edges.
+    const int rw = il % RW;
+    R fxr[3] = {R(0), R(0), R(0)}, fyr[3] = {R(0), R(0), R(0)};
+    if (own) {
+        const R gz = gz_lag(gi, gj);
+        const int e = (il + 1) * EJ + jl + 1;
+        const R cl = msqrt(gz * s_h[e]);
+        face_flux_v(s_h[e], s_h[e + EJ], s_u[e], s_u[e + EJ], s_v[e], s_v[e + EJ], cl, msqrt(gz * s_h[e + EJ]), gz,
+                    fxr[0], fxr[1], fxr[2]);
+        // y face: normal velocity v (SWEHR.c:1203), fluxes stored swapped (SWEHR.c:1216-1218)
         R f0, f1, f2;
-        face_flux_v(s_h[el], s_h[er], s_u[el], s_u[er], s_v[el], s_v[er], cl, msqrt(gz * s_h[er]), gz, f0, f1, f2);
-        const int f = fi * TJ + fj;
-        s_fx[f] = f0;
-        s_fx[NFX + f] = f1;
-        s_fx[2 * NFX + f] = f2;
-    };
-    // y face between ext cells el (lower) and el + 1; normal velocity v (SWEHR.c:1203)
-    auto yface_g = [&](int fi, int fj, R gz, R cl) {
-        const int el = (fi + 1) * EJ + fj, er = el + 1;
-        R f0, f1, f2;
-        face_flux_v(s_h[el], s_h[er], s_v[el], s_v[er], s_u[el], s_u[er], cl, msqrt(gz * s_h[er]), gz, f0, f1, f2);
-        const int f = fi * (TJ + 1) + fj;
-        s_fy[f] = f0;
-        s_fy[NFY + f] = f2;      // hu flux (tangential)
-        s_fy[2 * NFY + f] = f1;  // hv flux (normal), SWEHR.c:1216-1218
-    };
-    {
-        // the cell's own faces i+1/2 and j+1/2 share its lagged GZ and celerity
-        if (own) {
-            const R gz = gz_lag(gi, gj);
-            const R cl = msqrt(gz * s_h[(il + 1) * EJ + jl + 1]);
-            xface_g(il + 1, jl, gz, cl);
-            yface_g(il, jl + 1, gz, cl);
-        }
+        face_flux_v(s_h[e], s_h[e + 1], s_v[e], s_v[e + 1], s_u[e], s_u[e + 1], cl, msqrt(gz * s_h[e + 1]), gz, f0,
+                    f1, f2);
+        fyr[0] = f0;
+        fyr[1] = f2;  // hu flux (tangential)
+        fyr[2] = f1;  // hv flux (normal)
+        if (rw == RW - 1 && il < TI - 1)
+#pragma unroll
+            for (int m = 0; m < 3; m++) s_fxw[(m * (NFXW / TJ) + (il + 1) / RW) * TJ + jl] = fxr[m];
     }
     {
         // faces on the tile's low edges belong to the cell outside (or to the ghost, which
@@ -672,19 +686,26 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
                 face_flux_v(s_h[el], s_h[er], un[el], un[er], ut[el], ut[er], msqrt(gz * s_h[el]), msqrt(gz * s_h[er]),
                             gz, f0, f1, f2);
                 if (isx) {
-                    s_fx[fj] = f0;
-                    s_fx[NFX + fj] = f1;
-                    s_fx[2 * NFX + fj] = f2;
+                    s_fxw[fj] = f0;
+                    s_fxw[NFXW + fj] = f1;
+                    s_fxw[2 * NFXW + fj] = f2;
                 } else {
-                    const int f = fi * (TJ + 1);
-                    s_fy[f] = f0;
-                    s_fy[NFY + f] = f2;      // hu flux (tangential)
-                    s_fy[2 * NFY + f] = f1;  // hv flux (normal), SWEHR.c:1216-1218
+                    s_fye[fi] = f0;
+                    s_fye[NFYE + fi] = f2;      // hu flux (tangential)
+                    s_fye[2 * NFYE + fi] = f1;  // hv flux (normal), SWEHR.c:1216-1218
                 }
             }
         }
     }
     __syncthreads();
+    // the faces i-1/2 and j-1/2 (lane exchanges: every thread takes part)
+    R fxl[3], fyl[3];
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+        const R up = __shfl_up(fxr[m], TJ, 64), dn = lane_below(fyr[m]);
+        fxl[m] = rw > 0 ? up : s_fxw[(m * (NFXW / TJ) + il / RW) * TJ + jl];
+        fyl[m] = jl > 0 ? dn : s_fye[m * NFYE + il];
+    }
 
     // ---- phase 3: sources and update of the tile's own cells.  With more than KS
     // classes every thread of the block takes part (the second chunk of classes is
@@ -717,14 +738,12 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         ci.syn = R(slope_y<ZJ>(s_z, zi, zj, gj, ny, odxd));
         ci.sx_up = du ? R(slope_x<ZJ>(s_zp, zi + 1, zj, gi + 1, nx, odxd)) : ci.sxn;
         ci.sx_dn = dd ? R(slope_x<ZJ>(s_z, zi - 1, zj, gi - 1, nx, odxd)) : ci.sxn;
-        const int fxl = il * TJ + jl, fxr = fxl + TJ;
-        const int fyl = il * (TJ + 1) + jl, fyr = fyl + 1;
 #pragma unroll
         for (int m = 0; m < 3; m++) {
-            ci.fxl[m] = s_fx[m * NFX + fxl];
-            ci.fxr[m] = s_fx[m * NFX + fxr];
-            ci.fyl[m] = s_fy[m * NFY + fyl];
-            ci.fyr[m] = s_fy[m * NFY + fyr];
+            ci.fxl[m] = fxl[m];
+            ci.fxr[m] = fxr[m];
+            ci.fyl[m] = fyl[m];
+            ci.fyr[m] = fyr[m];
         }
         ci.totm_o = totm_o;
         ci.mv_o = mv_o;
