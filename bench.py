@@ -584,6 +584,18 @@ def main():
 
     roofline["hbm_copy_measured_GBs"] = round(hip.copy_bandwidth(), 1)
     log("device copy: %.1f GB/s" % roofline["hbm_copy_measured_GBs"])
+    # practical ceiling: the library's streaming kernel over the step's own planes (state
+    # 6 + 2S read and written, z read and written, the Ks raster read; z and Ks are FP64)
+    planes = 6 + 2 * S
+    if w == 8:
+        pin, pout = planes + 2, planes + 1
+    else:
+        pin, pout = planes + 4, planes + 2
+    stream_gbs = sv.hbm_stream(case.nx, case.ny, w, pin, pout)
+    roofline["hbm_stream_GBs"] = round(stream_gbs, 1)
+    roofline["frac_of_stream"] = round(achieved / stream_gbs, 4)
+    log("streaming kernel over the step's %d + %d planes: %.1f GB/s (step kernel at %.3f of it)"
+        % (pin, pout, stream_gbs, achieved / stream_gbs))
 
     cpu = None
     if not args.no_cpu:

@@ -225,6 +225,14 @@ int swehr_profile(swehr_handle *h, int nsteps, double *avg_ms, int *taken);
  * swehr_advance takes.  Diagnostic; no counterpart in the reference. */
 int swehr_profile_window(swehr_handle *h, int nsteps, double *avg_ms, int *taken);
 
+/* Diagnostic, no counterpart in the reference: the HBM rate (GB/s of bytes that must
+ * move) of a kernel that reads planes_in and writes planes_out planes of nx x ny cells
+ * of elem_bytes (4 or 8) each, laid out and traversed as the step kernel's state planes
+ * (rows at a pitch of a multiple of 64 cells, one row segment of 64 cells per wave,
+ * nontemporal loads and stores); best of reps timed launches on the current device.
+ * With the step's plane counts this is the step kernel's practical HBM ceiling. */
+int swehr_hbm_stream(int nx, int ny, int elem_bytes, int planes_in, int planes_out, int reps, double *gbs);
+
 /* Checkpoint of the instance's whole device state (snapshot/restore): *bytes is the
  * snapshot size; swehr_snapshot copies the state into host memory of that size, and
  * swehr_restore loads a snapshot taken from an instance created with the same

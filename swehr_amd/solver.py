@@ -77,6 +77,7 @@ def lib():
     L.swehr_snapshot.argtypes = [C.c_void_p, C.c_void_p]
     L.swehr_restore.argtypes = [C.c_void_p, C.c_void_p]
     L.swehr_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), P]
+    L.swehr_hbm_stream.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P]
     PP = C.POINTER(C.POINTER(C.c_double))
     L.swehr_batch_create.argtypes = [C.c_int, C.POINTER(_abi.Params), PP, PP, PP, PP, PP, C.c_int, C.c_int,
                                      C.POINTER(C.c_void_p)]
@@ -132,6 +133,15 @@ def tile_shapes():
         L.swehr_tile_shape(k, C.byref(a), C.byref(b))
         out.append((a.value, b.value))
     return out
+
+
+def hbm_stream(nx: int, ny: int, elem_bytes: int, planes_in: int, planes_out: int, reps: int = 5) -> float:
+    """GB/s that a kernel streaming planes_in + planes_out state planes of nx x ny cells
+    (the step kernel's layout and traversal) reaches on the current device."""
+    out = C.c_double(0)
+    check("swehr_hbm_stream", lib().swehr_hbm_stream(int(nx), int(ny), int(elem_bytes), int(planes_in),
+                                                     int(planes_out), int(reps), C.byref(out)))
+    return out.value
 
 
 def parameter_rasters(case):
