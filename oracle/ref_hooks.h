@@ -7,7 +7,10 @@
  * Every hook defaults to the reference's own hard-coded value, so with no REFH_*
  * variable set the hooked binary computes what the unmodified one does.  Hooks:
  *   REFH_KS / REFH_KS_FILE   saturated conductivity, scalar or i-major raster file (SWEHR.c:239)
- *   REFH_THETA0, REFH_THETAS, REFH_HF, REFH_ROUGHNESS   soil / friction values (SWEHR.c:237-242)
+ *   REFH_THETA0 / REFH_THETA0_FILE, REFH_THETAS / REFH_THETAS_FILE, REFH_HF / REFH_HF_FILE,
+ *   REFH_ROUGHNESS / REFH_ROUGHNESS_FILE
+ *                            soil moisture, suction head and Manning n0: scalar or i-major
+ *                            raster file, as for Ks (SWEHR.c:237-242)
  *   REFH_CSTABLE             target courant number (SWEHR.c:632)
  *   REFH_BNDRY="a,b,c,d"     boundary types (SWEHR.c:618-621)
  *   REFH_INFLOW=0            switch off the inflow at j=ny (SWEHR.c:1010-1035)
@@ -39,25 +42,32 @@ static double refh_env(const char *name, double dflt)
     return (s && *s) ? atof(s) : dflt;
 }
 
-static double refh_ks(int i, int j, int nx, int ny)
+/* Value at cell (i, j) (1-based) of a per-cell parameter: from the raster file named by
+ * the variable file_var (nx*ny numbers, i-major: row i holds the ny values of j) when it
+ * is set, else the scalar variable scalar_var, else the reference's value dflt.  slot
+ * caches one raster per parameter (0 Ks, 1 THETA0, 2 THETAS, 3 HF, 4 ROUGHNESS). */
+static double refh_raster(int slot, const char *file_var, const char *scalar_var, double dflt, int i, int j,
+                          int nx, int ny)
 {
-    static double *ras = NULL;
-    static int loaded = 0;
-    if (!loaded) {
-        const char *path = getenv("REFH_KS_FILE");
-        loaded = 1;
+    static double *ras[5] = {NULL, NULL, NULL, NULL, NULL};
+    static int loaded[5] = {0, 0, 0, 0, 0};
+    if (!loaded[slot]) {
+        const char *path = getenv(file_var);
+        loaded[slot] = 1;
         if (path && *path) {
             FILE *f = fopen(path, "r");
             if (!f) { fprintf(stderr, "refh: cannot open %s\n", path); exit(2); }
-            ras = (double *)calloc((size_t)nx * ny, sizeof(double));
+            ras[slot] = (double *)calloc((size_t)nx * ny, sizeof(double));
             for (long n = 0; n < (long)nx * ny; n++)
-                if (fscanf(f, "%lf", &ras[n]) != 1) { fprintf(stderr, "refh: short ks raster\n"); exit(2); }
+                if (fscanf(f, "%lf", &ras[slot][n]) != 1) { fprintf(stderr, "refh: short raster %s\n", path); exit(2); }
             fclose(f);
         }
     }
-    if (ras) return ras[(long)(i - 1) * ny + (j - 1)];
-    return refh_env("REFH_KS", 0.0);
+    if (ras[slot]) return ras[slot][(long)(i - 1) * ny + (j - 1)];
+    return refh_env(scalar_var, dflt);
 }
+
+static double refh_ks(int i, int j, int nx, int ny) { return refh_raster(0, "REFH_KS_FILE", "REFH_KS", 0.0, i, j, nx, ny); }
 
 static int refh_bndry(int side, int dflt)
 {

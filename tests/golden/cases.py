@@ -98,6 +98,36 @@ def _cfl_abort() -> Case:
                 pfrac=np.array([1.0]), sidecar=dict(cstable=1.5), name="cfl_abort")
 
 
+def _param_rasters() -> Case:
+    """Per-cell Ks, suction head, soil moisture and Manning n0, each varying by +-25%
+    around its mean, on a grid with nx != ny (flipping any one raster along i moves the
+    final state's most affected field by 7e-2 to 4.5e-1).  theta_0 and theta_s are multiples of 2^-10, so
+    theta_s - theta_0 is exact and the case's dtheta raster equals the reference's
+    THETAS - THETA0 bit for bit.  Rain exceeds the infiltration capacity early, so no
+    cell recedes through the film-depth thresholds: the run's rounding floor is 3e-14."""
+    rng = np.random.default_rng(17)
+    nx, ny, dx = 7, 52, 0.5
+    i = np.arange(nx)[:, None]
+    j = np.arange(ny)[None, :]
+    topo = 0.12 * dx * j + 0.02 * dx * np.abs(i - 2) + 0.004 * rng.standard_normal((nx, ny))
+    rain = np.array([100 * MM_PER_H] * 5)
+
+    def around(mean):
+        return mean * (1 + 0.5 * (rng.random((nx, ny)) - 0.5))
+
+    ks = around(5 * MM_PER_H)
+    hf = around(0.01)
+    theta0 = np.round((0.1 + 0.025 * (rng.random((nx, ny)) - 0.5)) * 1024) / 1024
+    thetas = np.round((0.4 + 0.05 * (rng.random((nx, ny)) - 0.5)) * 1024) / 1024
+    n0 = around(0.05)
+    c = Case(nx=nx, ny=ny, dx=dx, tend=150.0, topo=topo, rain=rain, rint=30.0, ds=np.array([3e-5, 6e-4]),
+             pfrac=np.array([0.55, 0.45]), a=1200.0, ad=2500.0, F=0.01, J=25.0, ks_raster=ks, hf_raster=hf,
+             dtheta_raster=thetas - theta0, roughness_raster=n0,
+             sidecar=dict(cstable=0.4, inflow=0), movies=0, name="param_rasters")
+    c.theta_rasters = (theta0, thetas)
+    return c
+
+
 def _c1() -> Case:
     c = config_c1(seed=1)
     c.name = "c1_plane_100x40"
@@ -117,6 +147,8 @@ CASES = [
     GoldenCase("closed_basin", "hooked", _closed_basin, dump=True, note="all four sides walls"),
     GoldenCase("cfl_abort", "hooked", _cfl_abort, dump=True, note="courant number 1.5 -> CFL stop"),
     GoldenCase("c1_plane_100x40", "hooked", _c1, note="the 100 x 40 plane of the CPU configuration, 1200 s"),
+    GoldenCase("param_rasters", "hooked", _param_rasters, dump=True,
+               note="per-cell Ks, HF, THETA0, THETAS and Manning n0 rasters on a 7 x 52 grid"),
 ]
 
 BY_NAME = {c.name: c for c in CASES}
@@ -130,8 +162,26 @@ def hook_env(case: Case, d: str, maxsteps: int = 0, dump_path: str = "") -> dict
         env["REFH_KS_FILE"] = os.path.join(d, "ks.txt")
     elif "ks" in sc:
         env["REFH_KS"] = "%.17g" % sc["ks"]
+    # the per-cell rasters: the sidecar's hf.txt and roughness.txt are in the reference
+    # hooks' raster format; theta_s - theta_0 needs both rasters, written here from the
+    # pair the case was built from (their difference must be the case's dtheta exactly)
+    if case.hf_raster is not None:
+        env["REFH_HF_FILE"] = os.path.join(d, "hf.txt")
+    if case.roughness_raster is not None:
+        env["REFH_ROUGHNESS_FILE"] = os.path.join(d, "roughness.txt")
+    if case.dtheta_raster is not None:
+        theta0, thetas = case.theta_rasters
+        if not np.array_equal(thetas - theta0, case.dtheta_raster):
+            raise ValueError("theta_s - theta_0 differs from the case's dtheta raster")
+        for fn, r in (("theta0.txt", theta0), ("thetas.txt", thetas)):
+            with open(os.path.join(d, fn), "w") as fh:
+                fh.write("\t".join("%.17g" % v for v in np.asarray(r, dtype=np.float64).reshape(-1)) + "\n")
+        env["REFH_THETA0_FILE"] = os.path.join(d, "theta0.txt")
+        env["REFH_THETAS_FILE"] = os.path.join(d, "thetas.txt")
     for key, var in (("theta0", "REFH_THETA0"), ("thetas", "REFH_THETAS"), ("hf", "REFH_HF"),
                      ("roughness", "REFH_ROUGHNESS"), ("cstable", "REFH_CSTABLE")):
+        if var + "_FILE" in env or (key in ("theta0", "thetas") and case.dtheta_raster is not None):
+            continue
         if key in sc:
             env[var] = "%.17g" % sc[key]
     if "inflow" in sc:

@@ -10,6 +10,7 @@ above the error measured on the MI355X (the FP64 path is deterministic: every
 reduction has a fixed order, and the one atomic is a max, which is exact).  Time and
 step counts are compared as well.
 """
+import json
 import os
 
 import numpy as np
@@ -47,7 +48,8 @@ def _pair(case, steps, precision=_abi.FP64):
 
 
 SHORT = [("shipped", 1), ("shipped", 40), ("inflow_mix", 1), ("inflow_mix", 300), ("narrow", 200),
-         ("rain_plane", 120), ("transmissive", 150), ("closed_basin", 150), ("cfl_abort", 60)]
+         ("rain_plane", 120), ("transmissive", 150), ("closed_basin", 150), ("cfl_abort", 60),
+         ("param_rasters", 300)]
 
 
 @pytest.mark.parametrize("name,steps", SHORT, ids=["%s-%d" % x for x in SHORT])
@@ -294,3 +296,42 @@ def test_parameter_rasters_match_oracle(precision, tol):
     k, e = worst(errs)
     print("worst %s %.3e" % (k, e))
     assert e <= tol, (k, e)
+
+
+def _dump_state(d, nclass):
+    """The reference's full-precision dump (tests/golden/<case>/dump.npz) as a state dict."""
+    st = {k: d[k] for k in ("depth", "uh", "vh", "topo", "vinf", "totm", "dep", "totsed", "maxvel", "maxdepth",
+                            "velx", "vely")}
+    st["ch"] = np.stack([d["ch%d" % (k + 1)] for k in range(nclass)])
+    st["m"] = np.stack([d["m%d" % (k + 1)] for k in range(nclass)])
+    return st
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [_abi.FP64, _abi.FP32], ids=["fp64", "fp32"])
+def test_parameter_raster_golden_matches_reference(precision):
+    """Non-uniform Ks, suction head, theta_0 / theta_s and Manning n0 rasters on a 7 x 52
+    grid, whole run (600 steps), against the final state the reference itself wrote
+    (SWEHR.c built with the raster hooks; the rasters are read per cell where
+    SWEHR.c:237-242 sets them).  FP64: every field within RTOL_FP64_SHORT of the
+    reference; FP32: depth within 1e-4."""
+    name = "param_rasters"
+    case = _case(name)
+    with open(os.path.join(GOLD, name, "expected.json")) as fh:
+        exp = json.load(fh)
+    ref = _dump_state(np.load(os.path.join(GOLD, name, "dump.npz")), case.nclass)
+    print("case", name, "precision", precision, flush=True)
+    with sv.Solver(case, precision=precision) as s:
+        s.run()
+        info = s.info()
+        errs = compare_states(s.state(), ref, case.nclass)
+    print({k: "%.2e" % v for k, v in errs.items()})
+    assert info["done"] == 1
+    if precision == _abi.FP64:
+        assert info["steps"] == exp["steps"]
+        assert info["t"] == pytest.approx(exp["t"], rel=1e-12, abs=1e-12)
+        k, e = worst(errs)
+        assert e <= RTOL_FP64_SHORT, (k, e)
+    else:
+        assert abs(info["steps"] - exp["steps"]) <= max(2, exp["steps"] // 100)
+        assert errs["depth"] <= 1e-4, errs["depth"]
