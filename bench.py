@@ -320,15 +320,16 @@ def bench_precision(hip, grid: int, t_check: float, window: int):
             S_fields = {"depth": s.field("depth"), "speed": s.field("speed"), "stage": st}
             S_fields["sed"] = [s.field("sed", k + 1) for k in range(S)]
             res[name] = (S_fields, s.info(), s.budget())
-    # sensitivity of the FP64 solution itself: the same run with the topography
-    # perturbed by 1e-12 relative (discrete events in the step amplify any change)
-    import numpy as np
-
-    pert = copy.copy(case)
-    pert.topo = case.topo * (1 + 1e-12 * np.random.default_rng(7).standard_normal(case.topo.shape))
-    with sv.Solver(pert, precision=_abi.FP64) as s:
-        s.run()
-        floor = {"depth": s.field("depth"), "speed": s.field("speed")}
+    # sensitivity of the FP64 solution itself at FP32 resolution: the same run with the
+    # topography perturbed by 6e-8 relative (FP32's unit roundoff), and with the rain
+    # rates lowered by 6e-8 (discrete events in the step amplify any change)
+    floor = {"depth": 0.0, "speed": 0.0}
+    b64 = res["fp64"][0]
+    for pert in [perturbed_at_fp32_resolution(case, 7)[k] for k in (0, 3)]:
+        with sv.Solver(pert, precision=_abi.FP64) as s:
+            s.run()
+            floor["depth"] = max(floor["depth"], rel_l2(s.field("depth"), b64["depth"]))
+            floor["speed"] = max(floor["speed"], rel_l2(s.field("speed"), b64["speed"]))
     a, b = res["fp32"][0], res["fp64"][0]
     n = min(a["stage"].shape[1], b["stage"].shape[1])
     out["fp32_vs_fp64"] = {
@@ -339,9 +340,9 @@ def bench_precision(hip, grid: int, t_check: float, window: int):
         "rel_l2_outlet_sedigraph": rel_l2(a["stage"][3, :n], b["stage"][3, :n]),
         "steps": [res["fp32"][1]["steps"], res["fp64"][1]["steps"]]}
     out["fp64_sensitivity"] = {
-        "perturbation": "topography x (1 + 1e-12 N(0,1)), FP64",
-        "rel_l2_depth": rel_l2(floor["depth"], b["depth"]),
-        "rel_l2_speed": rel_l2(floor["speed"], b["speed"])}
+        "perturbation": "FP64 runs with the topography x (1 + 6e-8 N(0,1)), and with the rain x (1 - 6e-8) "
+                        "(FP32 resolution): the larger rel L2",
+        "rel_l2_depth": floor["depth"], "rel_l2_speed": floor["speed"]}
     for name in ("fp64", "fp32"):
         we, se = budget_errors(res[name][2])
         out["mass_balance_rel_%s" % name] = {"water": we, "sediment": se}
@@ -362,76 +363,172 @@ def bench_precision(hip, grid: int, t_check: float, window: int):
     return out
 
 
-def bench_batched(hip, count: int, window: int, scale_max: float):
-    """Ensemble of hillslopes: FP32 in one ensemble (every stage one launch over all
-    instances) against plain FP64 instances run one after another, same steps."""
+def perturbed_at_fp32_resolution(case, seed: int):
+    """The case with one input changed at FP32 resolution (6e-8 = 2^-24, FP32's unit
+    roundoff), in four ways: the topography and the Ks raster (or scalar Ks) multiplied
+    by (1 + 6e-8 N(0,1)), and the rain rates by (1 + 6e-8) and (1 - 6e-8)."""
+    import numpy as np
+
+    g = np.random.default_rng(seed).standard_normal(case.topo.shape)
+    out = []
+    p = copy.copy(case)
+    p.topo = case.topo * (1 + 6e-8 * g)
+    out.append(p)
+    p = copy.copy(case)
+    if case.ks_raster is not None:
+        p.ks_raster = case.ks_raster * (1 + 6e-8 * g)
+    else:
+        p.sidecar = dict(case.sidecar, ks=case.side("ks") * (1 + 6e-8))
+    out.append(p)
+    for sign in (1, -1):
+        p = copy.copy(case)
+        p.rain = case.rain * (1 + sign * 6e-8)
+        out.append(p)
+    return out
+
+
+def fp32_resolution_floor(case, depth64, seed: int) -> float:
+    """How far the FP64 depth moves when an input changes at FP32 resolution: the
+    largest rel L2 of the FP64 runs of perturbed_at_fp32_resolution.  Where the run
+    turns at the model's thresholds (wet/dry, the film depth, ponding onset), this is how
+    far any FP32 evaluation of it can be expected to land from FP64 (the FP32 form holds
+    the rain rate, the topography and the state at that resolution)."""
+    from swehr_amd import _abi
+    from swehr_amd import solver as sv
+
+    out = 0.0
+    for p in perturbed_at_fp32_resolution(case, seed):
+        with sv.Solver(p, precision=_abi.FP64) as s:
+            s.run()
+            out = max(out, rel_l2(s.field("depth"), depth64))
+    return out
+
+
+def bench_batched(hip, count: int, scale_max: float, tend: float, plain_every: int, floor_max: int):
+    """Ensemble of hillslopes, each run for its configured duration (tend): the FP32 form
+    in one ensemble (every stage one launch over all instances) against the plain FP64
+    step run instance after instance.  Accuracy: each member's FP32 depth against its
+    FP64 depth at tend (the FP64 reference comes from an FP64 ensemble, which steps every
+    member exactly as it steps alone).  Where a member's FP32 error exceeds 1e-4, its
+    FP32-resolution floor (fp32_resolution_floor) shows how far FP64 itself moves there
+    under input changes of FP32's size (up to floor_max members).  The plain FP64
+    throughput is timed on every plain_every-th member's whole run."""
     from swehr_amd import _abi
     from swehr_amd import solver as sv
     from swehr_amd.inputs import config_c4
-
-    cases = config_c4(count=count, scale_max=scale_max)
-    cells = sum(c.nx * c.ny for c in cases)
-    S = cases[0].nclass
-    warm = 20
     import numpy as np
 
-    depth32, mb32 = [], []
-    with sv.Batch(cases, precision=_abi.FP32) as b:
-        best, tile_ms = b.autotune(3)
-        b.advance(warm)
-        e0, e1 = hip.event(), hip.event()
+    cases = config_c4(count=count, scale_max=scale_max, tend=tend)
+    cells = sum(c.nx * c.ny for c in cases)
+    S = cases[0].nclass
+
+    def whole_run(b):
         hip.sync()
         t0 = time.perf_counter()
-        b.advance(window)
+        left = b.advance()
         hip.sync()
-        batch_s = time.perf_counter() - t0
-        steps = sum(b.member(i).info()["steps"] - warm for i in range(len(cases)))
-        for i in range(len(cases)):
-            m = b.member(i)
-            depth32.append(m.field("depth"))
-            mb32.append(budget_errors(m.budget()))
-    batch_value = sum(c.nx * c.ny for c in cases) * S * window / batch_s
-    plain_s = 0.0
-    plain_cc = 0
-    num = den = pnum = 0.0
-    worst_member = 0.0
-    for i, c in enumerate(cases):
-        with sv.Solver(c, precision=_abi.FP64) as s:
-            s.run(warm)
-            n, ms, wall, _ = timed_run(hip, s, window)
+        return time.perf_counter() - t0, left
+
+    with sv.Batch(cases, precision=_abi.FP32) as b:
+        best, tile_ms = b.autotune(3)
+        batch_s, left = whole_run(b)
+        if left:
+            raise RuntimeError("%d ensemble members did not finish" % left)
+        steps32 = [b.member(i).info()["steps"] for i in range(len(cases))]
+        depth32 = [b.member(i).field("depth") for i in range(len(cases))]
+        mb32 = [budget_errors(b.member(i).budget()) for i in range(len(cases))]
+    work32 = sum(c.nx * c.ny * S * n for c, n in zip(cases, steps32))
+    log("C4 FP32 ensemble: %d members to t = %.0f s in %.2f s" % (len(cases), tend, batch_s))
+    with sv.Batch(cases, precision=_abi.FP64) as b:
+        b64_s, left = whole_run(b)
+        steps64 = [b.member(i).info()["steps"] for i in range(len(cases))]
+        depth64 = [b.member(i).field("depth") for i in range(len(cases))]
+    log("C4 FP64 ensemble (reference depths): %.2f s" % b64_s)
+    plain_s, plain_cc, plain_n = 0.0, 0, 0
+    for i in range(0, len(cases), max(1, plain_every)):
+        with sv.Solver(cases[i], precision=_abi.FP64) as s:
+            n, ms, wall, _ = timed_run(hip, s, 1 << 40)
             plain_s += wall
-            plain_cc += c.nx * c.ny * S * n
-            d64 = s.field("depth")
-        # the same FP64 run with the topography perturbed by 1e-12 (relative): how much
-        # the FP64 depth itself moves under a perturbation far below FP32 rounding
-        pert = copy.copy(c)
-        pert.topo = c.topo * (1 + 1e-12 * np.random.default_rng(i).standard_normal(c.topo.shape))
-        with sv.Solver(pert, precision=_abi.FP64) as s:
-            s.run(warm + window)
-            dp = s.field("depth")
-        diff = float(np.sum((depth32[i] - d64) ** 2))
-        num += diff
-        pnum += float(np.sum((dp - d64) ** 2))
-        den += float(np.sum(d64 ** 2))
-        worst_member = max(worst_member, rel_l2(depth32[i], d64))
-    plain_value = plain_cc / plain_s
-    return {"workload": "%d hillslopes, %d cells total, S=%d, %d steps each after %d warm-up steps"
-                        % (len(cases), cells, S, window, warm),
-            "fp32_batched": {"value": batch_value, "wall_s": batch_s, "tile": best,
-                             "autotune_ms_per_step": [round(float(x), 4) for x in tile_ms]},
-            "fp64_plain_sequential": {"value": plain_value, "wall_s": plain_s},
-            "ratio": batch_value / plain_value, "member_steps": steps,
-            "fp32_vs_fp64_after_the_window": {
+            plain_cc += cases[i].nx * cases[i].ny * S * n
+            plain_n += 1
+            if not np.array_equal(s.field("depth"), depth64[i]):
+                raise RuntimeError("member %d: the FP64 ensemble differs from the plain FP64 run" % i)
+    log("C4 FP64 plain: %d members in %.2f s" % (plain_n, plain_s))
+    err = [rel_l2(a, r) for a, r in zip(depth32, depth64)]
+    num = sum(float(np.sum((a - r) ** 2)) for a, r in zip(depth32, depth64))
+    den = sum(float(np.sum(r ** 2)) for r in depth64)
+    floors = {}
+    for i in sorted(range(len(cases)), key=lambda k: -err[k]):
+        if err[i] <= 1e-4 or len(floors) >= floor_max:
+            break
+        floors[i] = fp32_resolution_floor(cases[i], depth64[i], i)
+    e = np.asarray(err)
+    over = [i for i in range(len(cases)) if err[i] > 1e-4]
+    return {"workload": "%d hillslopes, %d cells total, S=%d, each run for its configured %.0f s"
+                        % (len(cases), cells, S, tend),
+            "fp32_batched": {"value": work32 / batch_s, "wall_s": batch_s, "member_steps": int(sum(steps32)),
+                             "tile": best, "autotune_ms_per_step": [round(float(x), 4) for x in tile_ms]},
+            "fp64_plain_sequential": {"value": plain_cc / plain_s, "wall_s": plain_s,
+                                      "members_timed": plain_n, "every": plain_every},
+            "ratio": (work32 / batch_s) / (plain_cc / plain_s),
+            "fp32_vs_fp64_at_tend": {
                 "rel_l2_depth_all_members": (num / den) ** 0.5 if den > 0 else 0.0,
-                "rel_l2_depth_worst_member": worst_member,
-                "fp64_sensitivity_rel_l2_depth_all_members": (pnum / den) ** 0.5 if den > 0 else 0.0,
+                "rel_l2_depth_worst_member": float(e.max()),
+                "rel_l2_depth_median_member": float(np.median(e)),
+                "members_above_1e-4": int(np.sum(e > 1e-4)),
+                "rel_l2_depth_per_member": [float("%.2g" % x) for x in err],
+                "steps_fp32_minus_fp64_max": int(max(abs(a - b) for a, b in zip(steps32, steps64))),
+                "fp32_resolution_floor": {
+                    "what": "largest rel L2 of FP64 depth when one input changes at FP32 resolution (topography "
+                            "or Ks raster x (1 + 6e-8 N(0,1)), rain x (1 +- 6e-8)), for the members above 1e-4",
+                    "per_member": {str(k): float("%.3g" % v) for k, v in floors.items()},
+                    "fp32_error_over_floor_max": max((err[k] / v for k, v in floors.items() if v > 0), default=None),
+                    "members_above_1e-4_without_floor": [k for k in over if k not in floors]},
                 "mass_balance_rel_worst_member": {"water": max(x[0] for x in mb32),
                                                   "sediment": max(x[1] for x in mb32)}}}
 
 
-def bench_serving(hip, nreq: int, rate: float, time_scale: float, mem_gib: float, slots: int, seed: int = 5):
-    """Poisson stream of solve requests into the continuous-batching server (FP32 state,
-    FP64 budgets); latency = completion - arrival (wall clock)."""
+def serve(srv, reqs, arrivals):
+    """Submit reqs at the given arrival times (seconds from now; all 0 for a closed batch)
+    and pump the server until every request has finished.  Returns (latencies in
+    seconds, completion minus arrival, per request; work in cell-class-steps; wall)."""
+    import numpy as np
+
+    n = len(reqs)
+    lat, work = {}, 0.0
+    t0 = time.perf_counter()
+    sub, nxt = {}, 0
+    while len(lat) < n:
+        now = time.perf_counter() - t0
+        while nxt < n and arrivals[nxt] <= now:
+            rid = srv.submit(reqs[nxt])
+            sub[rid] = (nxt, t0 + arrivals[nxt])
+            nxt += 1
+        running, queued, finished = srv.pump(16)
+        while True:
+            d = srv.next_done()
+            if d is None:
+                break
+            rid, info = d
+            k, ta = sub[rid]
+            lat[k] = time.perf_counter() - ta
+            c = reqs[k]
+            work += c.nx * c.ny * c.nclass * info["steps"]
+            srv.release(rid)
+        if running == 0 and queued == 0 and nxt < n:
+            time.sleep(max(0.0, min(0.01, t0 + arrivals[nxt] - time.perf_counter())))
+    return np.array([lat[k] for k in range(n)]), work, time.perf_counter() - t0
+
+
+def bench_serving(hip, nreq: int, load: float, cap_requests: int, mem_gib: float, slots: int, seed: int = 5):
+    """Solve requests of the configured mix (square grids 128^2-2048^2, 3-10 classes,
+    durations U[600, 3600] s of simulated time) through the continuous-batching server
+    (FP32 state, FP64 budgets).  Capacity first: cap_requests requests of the same mix
+    submitted at once, work completed over the makespan (a lower bound of the saturated
+    rate: the makespan includes the drain, when fewer requests are left than slots).
+    Then an open loop: nreq requests arriving as a Poisson stream at load x capacity,
+    in requests per second of the capacity run's mean work per request; latency =
+    completion - arrival (wall clock)."""
     import numpy as np
 
     from swehr_amd import _abi
@@ -439,43 +536,29 @@ def bench_serving(hip, nreq: int, rate: float, time_scale: float, mem_gib: float
     from swehr_amd.inputs import config_c5_request
 
     rng = np.random.default_rng(seed)
-    reqs = []
-    for k in range(nreq):
-        c = config_c5_request(rng, k)
-        c.tend = max(10.0, c.tend * time_scale)
-        reqs.append(c)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
-    lat, work = {}, 0.0
+    cap_reqs = [config_c5_request(rng, 10000 + k) for k in range(cap_requests)]
+    reqs = [config_c5_request(rng, k) for k in range(nreq)]
     with sv.Server(mem_bytes=int(mem_gib * 2 ** 30), max_slots=slots, precision=_abi.FP32) as srv:
-        t0 = time.perf_counter()
-        sub = {}
-        nxt = 0
-        while len(lat) < nreq:
-            now = time.perf_counter() - t0
-            while nxt < nreq and arrivals[nxt] <= now:
-                rid = srv.submit(reqs[nxt])
-                sub[rid] = (nxt, t0 + arrivals[nxt])
-                nxt += 1
-            running, queued, finished = srv.pump(16)
-            while True:
-                d = srv.next_done()
-                if d is None:
-                    break
-                rid, info = d
-                k, ta = sub[rid]
-                lat[k] = time.perf_counter() - ta
-                c = reqs[k]
-                work += c.nx * c.ny * c.nclass * info["steps"]
-                srv.release(rid)
-            if running == 0 and queued == 0 and nxt < nreq:
-                time.sleep(max(0.0, min(0.01, t0 + arrivals[nxt] - time.perf_counter())))
-        wall = time.perf_counter() - t0
-    L = np.array([lat[k] for k in range(nreq)])
-    return {"workload": "%d requests, Poisson rate %.2f/s, sizes 128^2-2048^2 (weights .4/.3/.15/.1/.05), "
-                        "S 3-10, durations U[600,3600] s x %.2f" % (nreq, rate, time_scale),
+        _, cap_work, cap_wall = serve(srv, cap_reqs, np.zeros(cap_requests))
+        capacity = cap_work / cap_wall
+        per_req = cap_work / cap_requests
+        rate = load * capacity / per_req
+        log("C5 capacity: %d requests in %.2f s, %.3e cell-class-steps/s; open loop at %.3f requests/s"
+            % (cap_requests, cap_wall, capacity, rate))
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+        L, work, wall = serve(srv, reqs, arrivals)
+    sizes = sorted(set(c.nx for c in reqs))
+    return {"workload": "%d requests, Poisson at %.3f/s = %.2f x measured capacity, sizes %s (weights "
+                        ".4/.3/.15/.1/.05), S 3-10, durations U[600,3600] s (unscaled)"
+                        % (nreq, rate, load, "/".join("%d^2" % n for n in sizes)),
+            "capacity": {"requests": cap_requests, "wall_s": cap_wall, "cell_class_steps_per_s": capacity,
+                         "mean_work_per_request": per_req, "requests_per_s": capacity / per_req},
+            "arrival_rate_per_s": rate, "load": load,
             "throughput_cell_class_steps_per_s": work / wall, "wall_s": wall,
+            "mean_duration_s": float(np.mean([c.tend for c in reqs])),
             "latency_p50_s": float(np.percentile(L, 50)), "latency_p99_s": float(np.percentile(L, 99)),
-            "latency_mean_s": float(L.mean()), "mem_budget_gib": mem_gib, "slots": slots}
+            "latency_mean_s": float(L.mean()), "latency_max_s": float(L.max()),
+            "mem_budget_gib": mem_gib, "slots": slots}
 
 
 def main():
@@ -505,11 +588,16 @@ def main():
     ap.add_argument("--c3-check", type=float, default=900.0, help="simulated seconds for the FP32-vs-FP64 check")
     ap.add_argument("--c3-window", type=int, default=100)
     ap.add_argument("--c4-count", type=int, default=256)
-    ap.add_argument("--c4-window", type=int, default=100)
+    ap.add_argument("--c4-tend", type=float, default=1800.0, help="simulated seconds each C4 member runs")
+    ap.add_argument("--c4-plain-every", type=int, default=8,
+                    help="time the plain FP64 runs of every n-th C4 member")
+    ap.add_argument("--c4-floor-max", type=int, default=32,
+                    help="most C4 members whose FP32-resolution floor is measured (those with FP32 error > 1e-4)")
     ap.add_argument("--c4-scale-max", type=float, default=16.0)
     ap.add_argument("--c5-requests", type=int, default=40)
-    ap.add_argument("--c5-rate", type=float, default=4.0)
-    ap.add_argument("--c5-time-scale", type=float, default=0.05)
+    ap.add_argument("--c5-load", type=float, default=0.7, help="C5 arrival rate as a fraction of measured capacity")
+    ap.add_argument("--c5-capacity-requests", type=int, default=24,
+                    help="requests submitted at once to measure the server's capacity")
     ap.add_argument("--c5-mem-gib", type=float, default=64.0)
     ap.add_argument("--c5-slots", type=int, default=32)
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
@@ -631,11 +719,12 @@ def main():
         forms["precision_c3"] = bench_precision(hip, args.c3_grid, args.c3_check, args.c3_window)
         log("precision: %s" % json.dumps(forms["precision_c3"]))
     if "batched" in wanted:
-        forms["batched_c4"] = bench_batched(hip, args.c4_count, args.c4_window, args.c4_scale_max)
+        forms["batched_c4"] = bench_batched(hip, args.c4_count, args.c4_scale_max, args.c4_tend, args.c4_plain_every,
+                                            args.c4_floor_max)
         log("batched: %s" % json.dumps(forms["batched_c4"]))
     if "serving" in wanted:
-        forms["serving_c5"] = bench_serving(hip, args.c5_requests, args.c5_rate, args.c5_time_scale, args.c5_mem_gib,
-                                            args.c5_slots)
+        forms["serving_c5"] = bench_serving(hip, args.c5_requests, args.c5_load, args.c5_capacity_requests,
+                                            args.c5_mem_gib, args.c5_slots)
         log("serving: %s" % json.dumps(forms["serving_c5"]))
 
     wres = sv.water_residual(bud)

@@ -85,7 +85,7 @@ __device__ __forceinline__ void cell_begin(Cell<R> &X, const StepArgs<R> &A, con
 
     // mixture density from the previous update (SWEHR.c:1578-1579)
     R rho = rhow * (R(1) - ts) + rhos * ts;
-    if (rho > R(0.8) * rhos) rho = R(0.8) * rhos;
+    if (rho > thr_gt<R>(0.8 * kRhos)) rho = R(0.8) * rhos;
 
     // variable-density forcing (SWEHR.c:1282-1290); the same factor in x and y
     R gmx = R(0), gmy = R(0);
@@ -123,7 +123,7 @@ __device__ __forceinline__ void cell_begin(Cell<R> &X, const StepArgs<R> &A, con
 
     // debris-flow forcing (SWEHR.c:1314-1320)
     R dfm = mfmin(R(1), (ts - R(0.2)) * R(5));
-    if (h < R(9) * h0 || ts < R(0.2)) dfm = R(0);
+    if (h < A.t9h0_lt || ts < thr_lt<R>(0.2)) dfm = R(0);
     const R bt = -(R(1) - R(kLambda)) * gz * h * kfa;
     const R btx = hu < R(0) ? R(-1) * bt : bt;
     const R bty = hv < R(0) ? R(-1) * bt : bt;
@@ -132,18 +132,18 @@ __device__ __forceinline__ void cell_begin(Cell<R> &X, const StepArgs<R> &A, con
     // depth normal to the bed h / cos(atan|slope|) = h * msqrt(1 + slope^2)
     const R totm_old = in.totm_o;
     const R hvert = h * (n2 * rn);
-    const bool deep_v = hvert > h0;
+    const bool deep_v = hvert > A.th0_gt;
     const R ihv = deep_v ? mrcp(hvert) : R(0);
     const R mtstar = deep_v ? R(A.mtstar0) * (h0 * ihv) : R(A.mtstar0);
     const R H = mfmin(R(1), totm_old * mrcp(mtstar));  // mtstar = 0: NaN, which fmin drops as inf/NaN
     const R shield = deep_v ? h0 * ihv : R(1);
     const R rough = A.n_raster ? R(at(gmem(A.n_raster), c8L)) : R(A.roughness);
     X.rough = rough;
-    const bool active = h > R(10) * hfilm && ts < R(0.6);
+    const bool active = h > thr_gt<R>(10 * kHfilm) && ts < thr_lt<R>(0.6);
     const R oneh = R(1) - mfmin(ts, R(0.99));
     X.hind = (oneh * oneh) * (oneh * oneh);
-    const bool shutoff = h > R(9) * h0 && ts > R(kCthreshold);
-    X.shallow_v = h < R(9) * h0;
+    const bool shutoff = h > A.t9h0_gt && ts > thr_gt<R>(kCthreshold);
+    X.shallow_v = h < A.t9h0_lt;
     X.vshal = mfmax(h - R(3) * h0, R(0)) * A.inv_6h0;
     X.cg = R(1.09) * R(kRhos / kRhow - 1) * gz;
     // rainsplash and stream-power factors; class k multiplies by P_k or M_k
@@ -156,8 +156,8 @@ __device__ __forceinline__ void cell_begin(Cell<R> &X, const StepArgs<R> &A, con
         omega = rho * gz * sf * qm;
         const R oneH = R(1) - H;
         f1 = oneH * R(A.a) * Rr * shield;
-        e2ok = totm_old > R(1e-8);
-        e4ok = totm_old > R(1e-6);
+        e2ok = totm_old > thr_gt<R>(1e-8);
+        e4ok = totm_old > thr_gt<R>(1e-6);
         const R itm = e2ok ? mrcp(totm_old) : R(0);
         f2 = H * itm * R(A.ad) * Rr * shield;
         f3 = oneH * R(A.F) / R(A.J) * omega;
@@ -283,8 +283,8 @@ __device__ __forceinline__ void cell_class(Cell<R> &X, const StepArgs<R> &A, int
     // E is 0 as the reference's guard makes it)
     const R dk = R(A.ds[k]), pk = R(A.pf[k]);
     const R e1 = pk * X.f1;
-    const R e2 = (X.e2ok && m > R(1e-8)) ? m * X.f2 : R(0);
-    const bool ent = X.omega > R(0) && X.h > dk;
+    const R e2 = (X.e2ok && m > thr_gt<R>(1e-8)) ? m * X.f2 : R(0);
+    const bool ent = X.omega > R(0) && X.h > A.tds_gt[k];
     const R e3 = ent ? pk * X.f3 : R(0);
     const R e4 = (ent && X.e4ok && m > R(0)) ? m * X.f4 : R(0);
     // settling velocity sqrt(ak^2 + cg d) - ak (SWEHR.c:435).  For fine classes
@@ -306,7 +306,7 @@ __device__ __forceinline__ void cell_class(Cell<R> &X, const StepArgs<R> &A, int
     // exchange with the deposited layer
     // the running depth: in FP32 the depth before the step plus the increments so far
     const R hcur = sizeof(R) == sizeof(double) ? X.hn : (X.film ? hfilm : X.h) + X.dh_rest;
-    if (hcur < R(10) * hfilm) {
+    if (hcur < thr_lt<R>(10 * kHfilm)) {
         if constexpr (sizeof(R) == sizeof(double)) m += dt * e1 + u;
         else m = add_unbiased(m, dt * e1 + u, cell_next_bits(X));
         X.bud[B_DET] += (double)(dt * e1);

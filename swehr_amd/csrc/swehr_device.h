@@ -76,9 +76,14 @@ struct StepArgs {
     double ds[kMaxClasses], pf[kMaxClasses];
     // per-class settling constants a_k = 13.95 nu / d_k and a_k^2 (SWEHR.c:435)
     double ak[kMaxClasses], ak2[kMaxClasses];
-    // bit k: d_k > 0.0001, resp. d_k > 0.002, compared in this precision (the
-    // shallow-flow settling rules, SWEHR.c:1385-1388)
+    // bit k: d_k > 0.0001, resp. d_k > 0.002, compared in FP64 as the reference
+    // compares them (the shallow-flow settling rules, SWEHR.c:1385-1388)
     unsigned coarse1, coarse2;
+    // thresholds formed from parameters, rounded as thr_lt / thr_gt round: 9 h0 for
+    // h < 9 h0 and h > 9 h0 (SWEHR.c:1384-1391, 1414), h0 for the depth normal to the
+    // bed (SWEHR.c:1331, 1347), d_k for the entrainment condition h > d_k
+    R t9h0_lt, t9h0_gt, th0_gt;
+    R tds_gt[kMaxClasses];
     // quotients of parameters, divided on the host exactly as the step would (IEEE
     // division on both sides): 1/dx, and 1/(6 h0) in this precision
     double inv_dx;

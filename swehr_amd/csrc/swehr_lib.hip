@@ -424,9 +424,13 @@ void fill_args(swehr_handle *h, StepArgs<R> &A, const double *ds, const double *
         A.pf[k] = k < h->S ? pf[k] : 0.0;
         A.ak[k] = 13.95 * kNu / A.ds[k];
         A.ak2[k] = A.ak[k] * A.ak[k];
-        if ((R)A.ds[k] > (R)0.0001) A.coarse1 |= 1u << k;
-        if ((R)A.ds[k] > (R)0.002) A.coarse2 |= 1u << k;
+        if (A.ds[k] > 0.0001) A.coarse1 |= 1u << k;
+        if (A.ds[k] > 0.002) A.coarse2 |= 1u << k;
+        A.tds_gt[k] = thr_gt<R>(A.ds[k]);
     }
+    A.t9h0_lt = thr_lt<R>(9 * p.h0);
+    A.t9h0_gt = thr_gt<R>(9 * p.h0);
+    A.th0_gt = thr_gt<R>(p.h0);
 }
 
 template <typename R>

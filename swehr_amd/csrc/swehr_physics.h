@@ -12,6 +12,34 @@
 
 namespace swehr {
 
+// Thresholds of the reference's FP64 tests in the precision R.  A threshold t is formed
+// in FP64 as SWEHR.c forms it (a constant, a parameter or a product of them); the FP32
+// form rounds it down for x > t and up for x < t (x <= t is !(x > t)), to the float at
+// which every float x compares as (double)x does with t.  Rounded to nearest, a float
+// equal to that rounded value could compare on the other side of t.  In FP64 both are t.
+constexpr float f32_below(double t)  // largest float <= t, for finite t >= 0
+{
+    const float f = (float)t;
+    return (double)f > t ? __builtin_bit_cast(float, __builtin_bit_cast(unsigned, f) - 1u) : f;
+}
+constexpr float f32_above(double t)  // smallest float >= t, for finite t >= 0
+{
+    const float f = (float)t;
+    return (double)f < t ? __builtin_bit_cast(float, __builtin_bit_cast(unsigned, f) + 1u) : f;
+}
+template <typename R>
+__host__ __device__ constexpr R thr_gt(double t)
+{
+    if constexpr (sizeof(R) == sizeof(double)) return t;
+    else return f32_below(t);
+}
+template <typename R>
+__host__ __device__ constexpr R thr_lt(double t)
+{
+    if constexpr (sizeof(R) == sizeof(double)) return t;
+    else return f32_above(t);
+}
+
 // Math helpers overloaded for both precisions, so FP32 code never promotes to FP64.
 // sqrt for x = 0 or x a normal positive number (every use in the step: g h, squared
 // speeds, settling discriminants): x rsqrt(x) refined by one Goldschmidt and one
@@ -220,6 +248,10 @@ __device__ __forceinline__ float mrcp(float x) { return __builtin_amdgcn_rcpf(x)
 
 // constants of SWEHR.c:489-494, 632-650
 constexpr double kG = 9.81;
+// The film depth: the FP32 form compares depths with it rounded to float (the value
+// it stores for a dry cell).  That rounded value lies below 1e-5 and the next float
+// above 1e-5, so for every float h, h > float(1e-5) exactly when h > 1e-5 in FP64: the
+// wet / dry tests agree with the reference's without widening h.
 constexpr double kHfilm = 1e-5;
 constexpr double kPhi = 0.4;
 constexpr double kRhos = 2600.0;
@@ -302,7 +334,7 @@ __device__ __forceinline__ R manning2_over_hpow(R h, R n0, const double *exp_tab
         static_assert(PI == 2 || PI == 3, "friction powers 2.333 and 3.333");
         const float l2 = __builtin_amdgcn_logf(h);
         float y = -0.333f * l2;
-        if (h <= float(kHcFriction)) y += float(2 * kDepthExp) * (l2 - float(kLog2HcFriction));
+        if (!(h > thr_gt<float>(kHcFriction))) y += float(2 * kDepthExp) * (l2 - float(kLog2HcFriction));
         const float ih = __builtin_amdgcn_rcpf(h);
         float ihp = ih * ih;
         if (PI == 3) ihp = ihp * ih;
