@@ -360,12 +360,20 @@ constexpr int kStepWaves = sizeof(R) == 4 ? (SPLIT ? 4 : 5) : 3;
 // one class ahead in the class loop); the 96-VGPR budget of 5 waves/SIMD spills from
 // 2 on.
 constexpr int kFp32MOwn = 1;
-// FP64 two-chunk variant: classes whose M_k is loaded up front (to 10 classes within
-// its register budget, the first chunk's)
-constexpr int kSplitMOwn = 10;
+// FP64 two-chunk variant: classes whose M_k is loaded up front, and whose ch_k stays in
+// registers (8 and 7: the most its 168-VGPR budget holds without a spill)
+constexpr int kSplitMOwn = 8;
+constexpr int kSplitOwn = 7;
 // FP64: classes whose ch_k and M_k of the own cell stay in registers from the halo
-// staging on (the rest are read one class ahead in the class loop)
-constexpr int kFp64Own = 7;
+// staging on (the rest are read one class ahead in the class loop, where the first of
+// them waits for memory): as many as the 168 VGPRs of 3 waves per SIMD hold without a
+// spill for the tile shape, one fewer in the ensemble kernel, which also holds its
+// work-map entry (4096^2, S = 10: 2.17 ms per step with 7, 2.10 ms with 10)
+template <int TI, int TJ, bool BATCH>
+constexpr int fp64_own()
+{
+    return (TI == 8 && TJ == 32) || (TI == 16 && TJ == 16) ? (BATCH ? 9 : 10) : (BATCH && TI == 4 && TJ == 32 ? 8 : 9);
+}
 // FP32 variant for many classes: classes whose ch_k and M_k of the own cell stay in
 // registers (within the 128-VGPR budget of 4 waves per SIMD)
 constexpr int kFp32WideOwn = 10;
@@ -424,7 +432,7 @@ __device__ __forceinline__ double lane_below(double v)
 // per CU instead of 5, it is compiled for 4 waves per SIMD and spends the registers
 // this frees on the own cell's ch_k and M_k of the first classes, which are then read
 // once (with the halo staging) instead of again in the class loop.
-template <typename R, int TI, int TJ, bool SPLIT>
+template <typename R, int TI, int TJ, bool SPLIT, int KOWN>
 __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsigned char *smem_raw)
 {
     using T = TileGeom<R, TI, TJ>;
@@ -490,7 +498,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     // again there they would come from memory, the halo having left L2 by then
     // (in FP32 only in the SPLIT variant: the register budget of 5 waves per SIMD has
     // no room)
-    constexpr int KO = sizeof(R) == sizeof(double) ? kFp64Own : (SPLIT ? kFp32WideOwn : 0);
+    constexpr int KO = sizeof(R) == sizeof(double) ? (SPLIT ? kSplitOwn : KOWN) : (SPLIT ? kFp32WideOwn : 0);
     R ch_own[KO > 0 ? KO : 1];
 #pragma unroll
     for (int k = 0; k < KO; k++) ch_own[k] = k < S ? at(chin + k * plane, cRL) : R(0);
@@ -899,7 +907,7 @@ __global__ __launch_bounds__(TI *TJ, (kStepWaves<R, SPLIT>)) void step_kernel(St
     const int b = blockIdx.x;
     const int tile = (b & 7) * A.blocks_per_xcd + (b >> 3);
     if (tile >= A.nblocks) return;
-    step_tile<R, TI, TJ, SPLIT>(A, tile, smem_raw);
+    step_tile<R, TI, TJ, SPLIT, fp64_own<TI, TJ, false>()>(A, tile, smem_raw);
 }
 
 // --------------------------------------------------------------------------- K3
@@ -1099,7 +1107,7 @@ __global__ __launch_bounds__(TI *TJ, (kStepWaves<R, SPLIT>)) void batch_step_ker
     const int k = (b & 7) * per_xcd + (b >> 3);
     if (k >= n) return;
     const TileRef r = map[k];
-    step_tile<R, TI, TJ, SPLIT>(args[r.inst], r.tile, smem_raw);
+    step_tile<R, TI, TJ, SPLIT, fp64_own<TI, TJ, true>()>(args[r.inst], r.tile, smem_raw);
 }
 
 template <typename R>
