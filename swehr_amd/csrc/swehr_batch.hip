@@ -295,8 +295,12 @@ struct Engine {
     }
 };
 
+// A member instance: created like a single one, then moved onto the engine's stream
+// (its own would only hold the create uploads; every later operation on a member, the
+// ensemble launches, downloads, snapshots, runs in order on one stream, and an ensemble
+// of 256 members keeps 1 stream instead of 257).
 int make_instance(const swehr_params *p, const double *topo, const double *ks, const double *ds, const double *pf,
-                  const double *rain, int precision, int tile, swehr_handle **out)
+                  const double *rain, int precision, int tile, hipStream_t engine_stream, swehr_handle **out)
 {
     swehr_handle *h = nullptr;
     int rc = swehr_create(p, topo, ks, ds, pf, rain, precision, &h);
@@ -306,6 +310,14 @@ int make_instance(const swehr_params *p, const double *topo, const double *ks, c
         swehr_destroy(h);
         return rc;
     }
+    if (hipStreamSynchronize(h->stream) != hipSuccess || hipStreamDestroy(h->stream) != hipSuccess) {
+        swehr_set_error("member stream: %s%s", hipGetErrorString(hipGetLastError()), "");
+        h->stream = nullptr;
+        swehr_destroy(h);
+        return SWEHR_ERR_HIP;
+    }
+    h->stream = engine_stream;
+    h->own_stream = false;
     h->member = true;
     *out = h;
     return SWEHR_OK;
@@ -349,7 +361,8 @@ extern "C" int swehr_batch_create(int n, const swehr_params *params, const doubl
     int rc = b->eng.init();
     for (int i = 0; i < n && rc == SWEHR_OK; i++) {
         swehr_handle *h = nullptr;
-        rc = make_instance(&params[i], topo[i], ks ? ks[i] : nullptr, ds[i], pfrac[i], rain[i], precision, t, &h);
+        rc = make_instance(&params[i], topo[i], ks ? ks[i] : nullptr, ds[i], pfrac[i], rain[i], precision, t,
+                           b->eng.stream, &h);
         if (rc == SWEHR_OK) b->all.push_back(h);
     }
     if (rc) {
@@ -691,7 +704,7 @@ extern "C" int swehr_server_pump(swehr_server *s, int steps, int *running, int *
         Request &r = s->queue.front();
         swehr_handle *h = nullptr;
         int rc = make_instance(&r.p, r.topo.data(), r.ks.empty() ? nullptr : r.ks.data(), r.ds.data(), r.pf.data(),
-                               r.rain.data(), s->precision, E.tile, &h);
+                               r.rain.data(), s->precision, E.tile, E.stream, &h);
         if (rc == SWEHR_ERR_OOM) break;  // the device is fuller than the budget says: retry later
         if (rc) return rc;
         if (r.snap) {

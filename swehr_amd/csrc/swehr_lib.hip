@@ -1194,7 +1194,7 @@ extern "C" int swehr_destroy(swehr_handle *h)
     if (h->graph) (void)hipGraphDestroy(h->graph);
     if (h->mem) (void)hipFree(h->mem);
     if (h->hctrl) (void)hipHostFree(h->hctrl);
-    if (h->stream) (void)hipStreamDestroy(h->stream);
+    if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return SWEHR_OK;
 }

@@ -52,6 +52,7 @@ struct swehr_handle {
     swehr::StepArgs<float> a32{};
     double *prast[3] = {nullptr, nullptr, nullptr};  // parameter rasters (device), separate allocations
     bool member = false;                              // owned by an ensemble or a server
+    bool own_stream = true;                           // false: the stream is the ensemble's or server's
 };
 
 namespace swehr {
