@@ -404,7 +404,8 @@ def fp32_resolution_floor(case, depth64, seed: int) -> float:
     return out
 
 
-def bench_batched(hip, count: int, scale_max: float, tend: float, plain_every: int, floor_max: int):
+def bench_batched(hip, count: int, scale_max: float, tend: float, plain_every: int, floor_max: int,
+                  persistent: bool = False):
     """Ensemble of hillslopes, each run for its configured duration (tend): the FP32 form
     in one ensemble (every stage one launch over all instances) against the plain FP64
     step run instance after instance.  Accuracy: each member's FP32 depth against its
@@ -412,7 +413,9 @@ def bench_batched(hip, count: int, scale_max: float, tend: float, plain_every: i
     member exactly as it steps alone).  Where a member's FP32 error exceeds 1e-4, its
     FP32-resolution floor (fp32_resolution_floor) shows how far FP64 itself moves there
     under input changes of FP32's size (up to floor_max members).  The plain FP64
-    throughput is timed on every plain_every-th member's whole run."""
+    throughput is timed on every plain_every-th member's whole run.  With persistent, the
+    FP32 ensemble's whole run is timed again in the persistent mode (one cooperative
+    launch per 16 steps) and in the graph mode, at the same 256-thread tile shape."""
     from swehr_amd import _abi
     from swehr_amd import solver as sv
     from swehr_amd.inputs import config_c4
@@ -439,6 +442,22 @@ def bench_batched(hip, count: int, scale_max: float, tend: float, plain_every: i
         mb32 = [budget_errors(b.member(i).budget()) for i in range(len(cases))]
     work32 = sum(c.nx * c.ny * S * n for c, n in zip(cases, steps32))
     log("C4 FP32 ensemble: %d members to t = %.0f s in %.2f s" % (len(cases), tend, batch_s))
+    pers = None
+    if persistent:
+        ptile = best if best in (1, 2, 3) else 1
+        walls = {}
+        for mode in (False, True):
+            with sv.Batch(cases, precision=_abi.FP32, tile=ptile) as b:
+                b.set_persistent(mode)
+                walls[mode], left = whole_run(b)
+                if mode:
+                    for i in range(len(cases)):
+                        if b.member(i).info()["steps"] != steps32[i] or not np.array_equal(b.member(i).field("depth"),
+                                                                                             depth32[i]):
+                            raise RuntimeError("member %d: persistent run differs from the graph run" % i)
+        pers = {"tile": ptile, "graph_wall_s": walls[False], "persistent_wall_s": walls[True],
+                "persistent_over_graph_time": walls[True] / walls[False]}
+        log("C4 FP32 persistent mode: %.2f s vs %.2f s graph (tile %d)" % (walls[True], walls[False], ptile))
     with sv.Batch(cases, precision=_abi.FP64) as b:
         b64_s, left = whole_run(b)
         steps64 = [b.member(i).info()["steps"] for i in range(len(cases))]
@@ -471,6 +490,7 @@ def bench_batched(hip, count: int, scale_max: float, tend: float, plain_every: i
             "fp64_plain_sequential": {"value": plain_cc / plain_s, "wall_s": plain_s,
                                       "members_timed": plain_n, "every": plain_every},
             "ratio": (work32 / batch_s) / (plain_cc / plain_s),
+            "persistent_mode": pers,
             "fp32_vs_fp64_at_tend": {
                 "rel_l2_depth_all_members": (num / den) ** 0.5 if den > 0 else 0.0,
                 "rel_l2_depth_worst_member": float(e.max()),
@@ -591,6 +611,8 @@ def main():
     ap.add_argument("--c4-tend", type=float, default=1800.0, help="simulated seconds each C4 member runs")
     ap.add_argument("--c4-plain-every", type=int, default=8,
                     help="time the plain FP64 runs of every n-th C4 member")
+    ap.add_argument("--c4-persistent", action="store_true",
+                    help="also time the C4 FP32 ensemble in the persistent mode (one cooperative launch per 16 steps)")
     ap.add_argument("--c4-floor-max", type=int, default=32,
                     help="most C4 members whose FP32-resolution floor is measured (those with FP32 error > 1e-4)")
     ap.add_argument("--c4-scale-max", type=float, default=16.0)
@@ -720,7 +742,7 @@ def main():
         log("precision: %s" % json.dumps(forms["precision_c3"]))
     if "batched" in wanted:
         forms["batched_c4"] = bench_batched(hip, args.c4_count, args.c4_scale_max, args.c4_tend, args.c4_plain_every,
-                                            args.c4_floor_max)
+                                            args.c4_floor_max, args.c4_persistent)
         log("batched: %s" % json.dumps(forms["batched_c4"]))
     if "serving" in wanted:
         forms["serving_c5"] = bench_serving(hip, args.c5_requests, args.c5_load, args.c5_capacity_requests,

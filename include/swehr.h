@@ -275,6 +275,12 @@ int swehr_batch_autotune(swehr_batch *b, int nsteps, int *best, double *ms_per_t
  * one with HIP events around each launch; avg_ms[0..2] receives the average duration
  * of the wave-speed, step and control launches (the steps advance the ensemble). */
 int swehr_batch_profile(swehr_batch *b, int nsteps, double *avg_ms);
+/* Persistent mode (on = 1), no counterpart in the reference: swehr_batch_advance runs
+ * the ensemble's steps as one cooperative launch per 16 steps (wave-speed pass, step and
+ * control of every instance separated by grid barriers) instead of a graph of three
+ * launches per step; results are bit-identical.  Needs a 256-thread tile shape (1-3):
+ * SWEHR_ERR_BAD_INPUT otherwise. */
+int swehr_batch_set_persistent(swehr_batch *b, int on);
 int swehr_batch_destroy(swehr_batch *b);
 
 /* ---- serving: continuous batching of solve requests.  Requests wait in a queue

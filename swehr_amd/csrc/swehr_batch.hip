@@ -11,6 +11,7 @@
 // The server adds continuous batching: requests wait in a FIFO and are admitted while
 // an instance slot is free and the device-memory budget holds their state; the work
 // map and the graph are rebuilt whenever the running set changes.
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -43,6 +44,65 @@ __global__ void set_limits_kernel(DevCtrl *const *ctrls, int n, long long add)
     if (i < n) {
         DevCtrl *c = ctrls[i];
         c->step_limit = add < 0 ? LLONG_MAX : c->steps + add;
+    }
+}
+
+// The ensemble's steps in ONE cooperative launch (the persistent mode of an ensemble):
+// every block loops over `steps` steps, and in each runs the wave-speed pass over the
+// (instance, tile) entries, a grid barrier, the fused step over the same entries, a grid
+// barrier, the control of the instances, a grid barrier.  The device functions are the
+// per-stage kernels' own, so the results are bit-identical to the three launches per
+// step.  Blocks take map entries in the XCD bands the per-stage kernels use (the
+// dispatcher deals block b to XCD b mod 8): block b serves band b mod 8 at positions
+// b / 8, b / 8 + grid / 8, ...  A block barrier separates consecutive tiles of a block,
+// whose shared memory the next tile reuses.  The grid holds at most as many blocks as
+// are resident at once (cooperative launch), so every grid barrier is reached.
+// Register cost: the step body inside a loop over tiles does not fit the 3-wave (FP64)
+// or 5-wave (FP32) budget that one tile per block fits without a spill; the compiler
+// spills ~420 bytes per lane here (and more with the stages as separate calls).
+template <typename R, int TI, int TJ, bool SPLIT>
+__global__ __launch_bounds__(TI *TJ, (kStepWaves<R, SPLIT>)) void batch_persistent_kernel(
+    const StepArgs<R> *__restrict__ args, const TileRef *__restrict__ map, int n, int per_xcd,
+    const int *__restrict__ insts, int ninst, int steps)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+    const int b = blockIdx.x, xcd = b & 7, bx = b >> 3, per_b = (int)(gridDim.x >> 3);
+    for (int s = 0; s < steps; s++) {
+        for (int q = bx; q < per_xcd; q += per_b) {
+            const int k = xcd * per_xcd + q;
+            if (k < n) {
+                const TileRef r = map[k];
+                wave_speed_tile<R, TI, TJ>(args[r.inst], r.tile);
+            }
+            __syncthreads();
+        }
+        grid.sync();
+        for (int q = bx; q < per_xcd; q += per_b) {
+            const int k = xcd * per_xcd + q;
+            if (k < n) {
+                const TileRef r = map[k];
+                step_tile<R, TI, TJ, SPLIT, fp64_own<TI, TJ, true>()>(args[r.inst], r.tile, smem_raw);
+            }
+            __syncthreads();
+        }
+        grid.sync();
+        for (int k = b; k < ninst; k += (int)gridDim.x) {
+            control_body(args[insts[k]]);
+            __syncthreads();
+        }
+        grid.sync();
+    }
+}
+
+template <typename R, bool SP>
+const void *persistent_kernel_t(int id)
+{
+    switch (id) {
+    case 1: return (const void *)&batch_persistent_kernel<R, 8, 32, SP>;
+    case 2: return (const void *)&batch_persistent_kernel<R, 4, 64, SP>;
+    case 3: return (const void *)&batch_persistent_kernel<R, 16, 16, SP>;
+    default: return nullptr;  // 128-thread tiles: not built in this mode
     }
 }
 
@@ -288,6 +348,52 @@ struct Engine {
         return SWEHR_OK;
     }
 
+    // Persistent mode: graph_steps steps in one cooperative launch instead of a graph
+    // of three launches per step, then the same gather of the control blocks.
+    bool persistent = false;
+
+    int run_persistent()
+    {
+        if (dirty) {
+            const int rc = rebuild();
+            if (rc) return rc;
+        }
+        const int n = (int)members.size();
+        if (!n) return SWEHR_OK;
+        const bool sp = batch_uses_split(precision, tile, max_classes, lds);
+        const void *k = precision == SWEHR_FP64
+                            ? (sp ? persistent_kernel_t<double, true>(tile) : persistent_kernel_t<double, false>(tile))
+                            : (sp ? persistent_kernel_t<float, true>(tile) : persistent_kernel_t<float, false>(tile));
+        if (!k) {
+            swehr_set_error("the persistent mode is built for the 256-thread tile shapes (1-3)%s%s", "", "");
+            return SWEHR_ERR_BAD_INPUT;
+        }
+        int dev = 0, coop = 0, cus = 0, per_cu = 0;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+        HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, lds));
+        const int grid = std::min(per_cu * cus, 8 * per_xcd) / 8 * 8;
+        if (!coop || grid < 8) {
+            swehr_set_error("no cooperative launch of the ensemble on this device%s%s", "", "");
+            return SWEHR_ERR_HIP;
+        }
+        const void *a = d_args;
+        const TileRef *m = d_map;
+        const int *ins = d_insts;
+        int nm = n_map, px = per_xcd, ni = n, st = graph_steps;
+        void *kargs[] = {(void *)&a, (void *)&m, (void *)&nm, (void *)&px, (void *)&ins, (void *)&ni, (void *)&st};
+        HIPCHK(hipLaunchCooperativeKernel(k, dim3(grid), dim3(256), kargs, (unsigned)lds, stream));
+        hipLaunchKernelGGL(gather_ctrl_kernel, dim3((n + 127) / 128), dim3(128), 0, stream, (DevCtrl *const *)d_ctrls,
+                           n, d_gather);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(h_gather, d_gather, members.size() * sizeof(DevCtrl), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        return SWEHR_OK;
+    }
+
+    int run() { return persistent ? run_persistent() : run_graph(); }
+
     bool member_idle(size_t i) const
     {
         const DevCtrl &c = h_gather[i];
@@ -387,7 +493,7 @@ extern "C" int swehr_batch_advance(swehr_batch *b, int64_t max_steps, int *remai
     int rc = E.set_limits((long long)max_steps);
     if (rc) return rc;
     while (!E.members.empty() && max_steps > 0) {
-        rc = E.run_graph();
+        rc = E.run();
         if (rc) return rc;
         std::vector<swehr_handle *> keep;
         bool any_running = false;
@@ -509,6 +615,17 @@ extern "C" int swehr_batch_profile(swehr_batch *b, int nsteps, double *avg_ms)
     double sum[3] = {0, 0, 0};
     if ((rc = E.profile_steps(nsteps, sum)) != SWEHR_OK) return rc;
     for (int k = 0; k < 3; k++) avg_ms[k] = sum[k] / nsteps;
+    return SWEHR_OK;
+}
+
+extern "C" int swehr_batch_set_persistent(swehr_batch *b, int on)
+{
+    if (!b) return SWEHR_ERR_BAD_INPUT;
+    if (on && (b->eng.tile < 1 || b->eng.tile > 3)) {
+        swehr_set_error("the persistent mode is built for the 256-thread tile shapes (1-3)%s%s", "", "");
+        return SWEHR_ERR_BAD_INPUT;
+    }
+    b->eng.persistent = on != 0;
     return SWEHR_OK;
 }
 

@@ -145,6 +145,11 @@ void launch_batch_wave(int id, int prec, int blocks, hipStream_t s, const void *
     else batch_wave_t(id, blocks, s, (const StepArgs<float> *)args, map, n, per_xcd);
 }
 
+bool batch_uses_split(int prec, int id, int max_classes, size_t lds)
+{
+    return prec == SWEHR_FP64 ? use_split<double>(id, max_classes, lds) : use_split<float>(id, max_classes, lds);
+}
+
 void launch_batch_step(int id, int prec, int blocks, size_t lds, hipStream_t s, const void *args, const TileRef *map,
                        int n, int per_xcd, int max_classes)
 {

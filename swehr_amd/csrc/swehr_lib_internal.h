@@ -70,4 +70,7 @@ void launch_batch_wave(int id, int prec, int blocks, hipStream_t s, const void *
 // kLdsClasses<double>)
 void launch_batch_step(int id, int prec, int blocks, size_t lds, hipStream_t s, const void *args, const TileRef *map,
                        int n, int per_xcd, int max_classes);
+// Whether tile shape id at precision prec runs the SPLIT step variant for max_classes
+// classes and lds bytes of tile LDS (the choice launch_batch_step makes).
+bool batch_uses_split(int prec, int id, int max_classes, size_t lds);
 }  // namespace swehr

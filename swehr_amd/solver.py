@@ -85,6 +85,7 @@ def lib():
     L.swehr_batch_handle.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
     L.swehr_batch_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), P]
     L.swehr_batch_profile.argtypes = [C.c_void_p, C.c_int, P]
+    L.swehr_batch_set_persistent.argtypes = [C.c_void_p, C.c_int]
     L.swehr_batch_destroy.argtypes = [C.c_void_p]
     L.swehr_server_create.argtypes = [C.c_int64, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
     L.swehr_server_submit.argtypes = [C.c_void_p, C.POINTER(_abi.Params), P, P, P, P, P, C.POINTER(C.c_int64)]
@@ -368,6 +369,11 @@ class Batch:
         ms = np.zeros(lib().swehr_tile_count())
         check("swehr_batch_autotune", lib().swehr_batch_autotune(self._b, int(nsteps), C.byref(best), dptr(ms)))
         return best.value, ms
+
+    def set_persistent(self, on: bool = True):
+        """Step in one cooperative launch per 16 steps instead of graphs of three
+        launches per step (bit-identical results)."""
+        check("swehr_batch_set_persistent", lib().swehr_batch_set_persistent(self._b, int(bool(on))))
 
     def profile(self, nsteps: int = 10):
         """Average durations (ms) of the ensemble's wave-speed, step and control launches

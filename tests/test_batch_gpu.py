@@ -276,3 +276,20 @@ def test_server_time_slicing_round_robins_requests_that_do_not_fit_together():
         assert max_running == 1
         assert srv.evictions() >= 2
 
+
+
+@pytest.mark.parametrize("precision", [_abi.FP64, _abi.FP32], ids=["fp64", "fp32"])
+def test_persistent_ensemble_equals_graph_ensemble(precision):
+    """The persistent mode (one cooperative launch per 16 steps, grid barriers between
+    the stages) runs the per-stage kernels' device functions in the same order: every
+    member ends bit-identical to the graph-launched ensemble's (and so to its run
+    alone), with the same per-instance step counts."""
+    cases = _mix()
+    print("persistent ensemble of", [(c.name, c.nx, c.ny, c.nclass) for c in cases], flush=True)
+    with sv.Batch(cases, precision=precision, tile=1) as a, sv.Batch(cases, precision=precision, tile=1) as b:
+        b.set_persistent(True)
+        assert a.advance() == 0
+        assert b.advance() == 0
+        for i, c in enumerate(cases):
+            assert a.member(i).info()["steps"] == b.member(i).info()["steps"], c.name
+            _same(a.member(i), b.member(i), c)
