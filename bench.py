@@ -638,7 +638,7 @@ def main():
         return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (the multi-GPU sharding "
-              "of the step is described in DESIGN.md)" % args.gpus, file=sys.stderr)
+              "of the step is a design note of this repository)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     from swehr_amd import _abi

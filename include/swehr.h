@@ -203,7 +203,8 @@ int swehr_budget_noalloc(swehr_handle *h, double *out);
  * interval so the caller can write the reference's progress line and movie frame
  * (SWEHR.c:1679-1698); 0 never pause.  steps_per_graph: time steps captured per HIP
  * graph launch (default 16; <= 0 keeps the current value).  tile: kernel tile shape
- * id (0 = automatic choice; see swehr_tile_count()). */
+ * id (0 = automatic choice; see swehr_tile_count()).  The step graph is captured and
+ * instantiated here, so the next swehr_advance() starts stepping at once. */
 int swehr_configure(swehr_handle *h, int pause_on_print, int steps_per_graph, int tile);
 
 /* Number of tile shapes the step kernel is built for, and the shape of id (1-based):

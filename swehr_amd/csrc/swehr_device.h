@@ -89,7 +89,9 @@ struct StepArgs {
     double inv_dx;
     R inv_6h0;
     DevCtrl *ctrl;
-    double *bud;       // [bud_slots][NB_TERMS][3]: running sum, compensation, last step's partial
+    // budget slots: [bud_slots][NB_TERMS][2] running sum and compensation, followed by
+    // [bud_slots][NB_TERMS] the last step's partials not yet folded into them (0 = none)
+    double *bud;
     double *stage;     // [kStageCols][stage_cap]
     double *saveflow;  // [kFlowCols][stage_cap]
     // outlet discharge: per column i the outflow through the j = 1/2 face in the last
@@ -224,15 +226,20 @@ __device__ __forceinline__ double slope_y(const double *P, int zi, int zj, int g
     return odx * (P[zi * ZJ + zj + 1] - P[zi * ZJ + zj]);
 }
 
-// Budget slot term p = {running sum, compensation, pending partial}: folds the pending
-// partial into the sum (Neumaier) and clears it.
-__device__ __forceinline__ void fold_budget(glob<double> *p)
+// Budget term g of an argument block's slots: folds its pending partial into the
+// running sum (Neumaier) and clears it.  Most partials are zero (the step kernel stores
+// only nonzero ones): then only the partial is read.
+__device__ __forceinline__ void fold_budget(glob<double> *bud, long long nterms, long long g)
 {
+    glob<double> *pend = bud + 2 * nterms + g;
+    const double x = *pend;
+    if (x == 0.0) return;
+    glob<double> *p = bud + 2 * g;
     double s = p[0], c = p[1];
-    neumaier(s, c, p[2]);
+    neumaier(s, c, x);
     p[0] = s;
     p[1] = c;
-    p[2] = 0.0;
+    *pend = 0.0;
 }
 
 }  // namespace swehr
@@ -322,9 +329,9 @@ __global__ __launch_bounds__(256) void wave_speed_kernel(StepArgs<R> A)
         }
     }
     // the step kernel's budget partials of the previous step
-    for (long long g = (long long)b * blockDim.x + threadIdx.x; g < (long long)A.bud_slots * NB_TERMS;
-         g += (long long)gridDim.x * blockDim.x)
-        fold_budget(gmem(A.bud) + 3 * g);
+    const long long nterms = (long long)A.bud_slots * NB_TERMS;
+    for (long long g = (long long)b * blockDim.x + threadIdx.x; g < nterms; g += (long long)gridDim.x * blockDim.x)
+        fold_budget(gmem(A.bud), nterms, g);
     block_max_to((unsigned long long *)C->smax_slots, b, (double)s);
 }
 
@@ -345,7 +352,8 @@ __device__ __forceinline__ void wave_speed_tile(const StepArgs<R> &A, int tile)
         s = wave_speed_cell(A, gmem(A.h[p]), gmem(A.hu[p]), gmem(A.hv[p]), gmem(A.z[(p3 + 2) % 3]), i, j, in);
     // the step kernel's budget partials of this tile's slots (previous step)
     if ((int)threadIdx.x < (TI * TJ / 64) * NB_TERMS)
-        fold_budget(gmem(A.bud) + 3 * ((size_t)tile * (TI * TJ / 64) * NB_TERMS + threadIdx.x));
+        fold_budget(gmem(A.bud), (long long)A.bud_slots * NB_TERMS,
+                    (long long)tile * (TI * TJ / 64) * NB_TERMS + threadIdx.x);
     block_max_to((unsigned long long *)C->smax_slots, tile, (double)s);
 }
 
@@ -882,10 +890,10 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         for (int k = 0; k < NB_TERMS; k++) bud[k] = X.bud[k];
     }
 
-    // ---- budget partials: each wave stores its sums of this step as the pending
-    // partial of its own slot (tile, wave); the wave-speed kernel of the next step folds
-    // it into the slot's compensated running sum (fold_budget).  No barrier and no
-    // read-modify-write at the end of the tile.
+    // ---- budget partials: each wave stores its nonzero sums of this step as the
+    // pending partials of its own slot (tile, wave); the wave-speed kernel of the next
+    // step folds them into the slot's compensated running sums (fold_budget) and clears
+    // them.  No barrier and no read-modify-write at the end of the tile.
     {
         double mine = 0;
 #pragma unroll
@@ -894,7 +902,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             if (__ballot(v != 0.0) != 0ull) v = wave_sum(v);  // most terms are zero in most tiles
             if (lane == k) mine = v;
         }
-        if (lane < NB_TERMS) gmem(A.bud)[(((size_t)tile * (NT / 64) + wid) * NB_TERMS + lane) * 3 + 2] = mine;
+        if (lane < NB_TERMS && mine != 0.0)
+            gmem(A.bud)[(size_t)A.bud_slots * NB_TERMS * 2 + ((size_t)tile * (NT / 64) + wid) * NB_TERMS + lane] = mine;
     }
 }
 

@@ -48,6 +48,14 @@ struct TileShape {
 constexpr TileShape kTiles[] = {{8, 32}, {4, 64}, {16, 16}, {4, 32}};
 constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
+// Waves per SIMD that the LDS of a block of `threads` admits: 160 KB per CU shared by
+// its 4 SIMDs, with 1 KB of static LDS per block (the step kernel's tables) beside the
+// dynamic size.
+inline int lds_waves_per_simd(size_t lds, int threads)
+{
+    return (int)((163840 / (lds + 1024)) * (size_t)(threads / 64) / 4);
+}
+
 template <typename R>
 size_t tile_lds(int id, int S)
 {
@@ -68,14 +76,6 @@ void launch_step_v(int id, int blocks, size_t lds, hipStream_t s, const StepArgs
     case 3: hipLaunchKernelGGL((step_kernel<R, 16, 16, SPLIT>), dim3(blocks), dim3(256), lds, s, A); break;
     default: hipLaunchKernelGGL((step_kernel<R, 4, 32, SPLIT>), dim3(blocks), dim3(128), lds, s, A); break;
     }
-}
-
-// Waves per SIMD that the LDS of a block of `threads` admits: 160 KB per CU shared by
-// its 4 SIMDs, with 1 KB of static LDS per block (the step kernel's tables) beside the
-// dynamic size.
-inline int lds_waves_per_simd(size_t lds, int threads)
-{
-    return (int)((163840 / (lds + 1024)) * (size_t)(threads / 64) / 4);
 }
 
 // The variant for many classes (SPLIT) where the default one does not serve them
@@ -253,8 +253,13 @@ __global__ __launch_bounds__(256) void budget_kernel(StepArgs<R> A, double *out)
     }
     for (long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x; b < A.bud_slots;
          b += (long long)gridDim.x * blockDim.x)
-        for (int k = 0; k < NB_TERMS; k++)
-            for (int m = 0; m < 3; m++) neumaier(s[5 + k], c[5 + k], A.bud[(b * NB_TERMS + k) * 3 + m]);
+        for (int k = 0; k < NB_TERMS; k++) {
+            // running sum, compensation, pending partial
+            const long long g = b * NB_TERMS + k;
+            neumaier(s[5 + k], c[5 + k], A.bud[2 * g]);
+            neumaier(s[5 + k], c[5 + k], A.bud[2 * g + 1]);
+            neumaier(s[5 + k], c[5 + k], A.bud[2 * (long long)A.bud_slots * NB_TERMS + g]);
+        }
     __shared__ double red[10][256];
     for (int n = 0; n < 10; n++) red[n][threadIdx.x] = s[n] + c[n];
     __syncthreads();
@@ -488,6 +493,8 @@ int build_graph(swehr_handle *h)
     }
     h->graph = g;
     HIPCHK(hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0));
+    // move the executable graph to the device now, not at its first launch
+    HIPCHK(hipGraphUpload(h->gexec, h->stream));
     return SWEHR_OK;
 }
 
@@ -691,8 +698,14 @@ extern "C" int swehr_configure(swehr_handle *h, int pause_on_print, int steps_pe
     if (pause_on_print == 0 || pause_on_print == 1) {
         *h->hflag = pause_on_print;
         HIPCHK(hipMemcpyAsync(&h->ctrl->pause_on_print, h->hflag, sizeof(int), hipMemcpyHostToDevice, h->stream));
-        HIPCHK(hipStreamSynchronize(h->stream));
     }
+    // a changed graph length or tile shape is captured and instantiated here, so that
+    // the next swehr_advance launches at once instead of first building the graph
+    if (!h->gexec) {
+        const int rc = build_graph(h);
+        if (rc) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
     return SWEHR_OK;
 }
 

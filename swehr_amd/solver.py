@@ -174,11 +174,16 @@ class Solver:
                             dptr(rain), int(precision), C.byref(h))
         check("swehr_create", rc)
         self._h = h
-        check("swehr_configure", L.swehr_configure(self._h, 1 if pause_on_print else 0, int(steps_per_graph),
-                                                   int(tile)))
-        for which, r in enumerate(parameter_rasters(case)):
-            if r is not None:
-                check("swehr_set_parameter_raster", L.swehr_set_parameter_raster(self._h, which, dptr(r)))
+        try:
+            check("swehr_configure", L.swehr_configure(self._h, 1 if pause_on_print else 0, int(steps_per_graph),
+                                                       int(tile)))
+            for which, r in enumerate(parameter_rasters(case)):
+                if r is not None:
+                    check("swehr_set_parameter_raster", L.swehr_set_parameter_raster(self._h, which, dptr(r)))
+        except SwehrError:
+            L.swehr_destroy(self._h)
+            self._h = None
+            raise
 
     def configure(self, pause_on_print: int = -1, steps_per_graph: int = 0, tile: int = 0):
         check("swehr_configure", lib().swehr_configure(self._h, int(pause_on_print), int(steps_per_graph), int(tile)))
