@@ -803,7 +803,8 @@ extern "C" int swehr_profile(swehr_handle *h, int nsteps, double *avg_ms, int *t
 // A snapshot is a header that records the instance's layout, then its whole device
 // block (which holds no pointers, so it restores into any instance of that layout).
 namespace {
-constexpr uint64_t kSnapMagic = 0x3150414e53524857ull;  // "WHRSNAP1"
+// version 2: budget slots as running sums followed by the pending partials
+constexpr uint64_t kSnapMagic = 0x3250414e53524857ull;  // "WHRSNAP2"
 constexpr size_t kSnapHeader = 256;
 struct SnapHeader {
     uint64_t magic;
