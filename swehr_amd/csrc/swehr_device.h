@@ -250,6 +250,26 @@ namespace swehr {
 
 // --------------------------------------------------------------------------- K1
 
+// Value held by the lane below (L - 1) by a DPP wave shift; lane 0 keeps its own.
+__device__ __forceinline__ int lane_below_i(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ float lane_below(float v) { return __int_as_float(lane_below_i(__float_as_int(v))); }
+__device__ __forceinline__ double lane_below(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = lane_below_i((int)(unsigned)b), hi = lane_below_i((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+// Value held by the lane above (L + 1) by a DPP wave shift; lane 63 keeps its own.
+__device__ __forceinline__ int lane_above_i(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xf, 0xf, false); }
+__device__ __forceinline__ float lane_above(float v) { return __int_as_float(lane_above_i(__float_as_int(v))); }
+__device__ __forceinline__ double lane_above(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = lane_above_i((int)(unsigned)b), hi = lane_above_i((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+
 // Wave-speed bound of the faces a cell owns: i+1/2 and j+1/2, plus 1/2 at i = 0 or
 // j = 0.  Ghost states follow the boundary rules of SWEHR.c:816-1035; GZ is that of
 // the previous step's flux pass, from the lagged bed zp (read from global memory).
@@ -302,30 +322,111 @@ __device__ __forceinline__ void block_max_to(unsigned long long *slots, int slot
     }
 }
 
-// Two cells per thread on 8 x 64 tiles (rows il and il + 4; a wave reads 64-cell row
-// segments of each field); tiles are dealt to XCDs in contiguous runs so that the
-// next row a tile reads is in the same XCD's L2.  The grid is the tile count rounded
-// up to a multiple of 8.
+// K1 on strips of rows: wave w of a block takes rows ib .. ib + K1_RPW - 1 (ib = first
+// row of the block + w K1_RPW) of 64 columns, lanes 0 and 63 holding the columns left
+// and right of the 62 whose faces it computes.  A cell's velocities (hu/h, hv/h) are
+// computed once and passed to the cells whose faces need them: the row above through
+// registers, the column to the left through a lane shift; each row of each field is
+// loaded once per wave.  Every face speed is the one wave_speed_cell computes (the same
+// velocities, celerities and GZ, the same boundary rules).  Tiles are dealt to XCDs in
+// contiguous runs; the grid is the tile count rounded up to a multiple of 8.
+constexpr int kK1Rows = 4;               // rows per wave
+constexpr int kK1TI = 4 * kK1Rows, kK1TJ = 62;  // rows and computed columns per block
+__host__ __device__ inline int k1_tiles(int nx, int ny) { return ((nx + kK1TI - 1) / kK1TI) * ((ny + kK1TJ - 1) / kK1TJ); }
+
 template <typename R>
 __global__ __launch_bounds__(256) void wave_speed_kernel(StepArgs<R> A)
 {
+    constexpr int RPW = kK1Rows;
     glob<DevCtrl> *C = gmem(A.ctrl);
     if (!step_active(C)) return;
     const long long steps = C->steps;
     const int p = (int)(steps & 1), p3 = (int)(steps % 3);
     const InflowState in = ctrl_inflow(C);
-    const int tiles_j = (A.ny + 63) >> 6, ntiles = ((A.nx + 7) >> 3) * tiles_j;
+    const int nx = A.nx, ny = A.ny;
+    const int tiles_j = (ny + kK1TJ - 1) / kK1TJ, ntiles = k1_tiles(nx, ny);
     const int b = blockIdx.x, per = (int)(gridDim.x >> 3);
     const int tile = (b & 7) * per + (b >> 3);
     R s = R(0);
     if (tile < ntiles) {
         const int ti = tile / tiles_j, tj = tile - ti * tiles_j;
-        const int i = ti * 8 + (int)(threadIdx.x >> 6), j = tj * 64 + (int)(threadIdx.x & 63);
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        const int j = tj * kK1TJ - 1 + lane;  // this lane's column (-1 or >= ny at the edges)
+        const int jc = j < 0 ? 0 : (j >= ny ? ny - 1 : j);
+        const bool comp = lane >= 1 && lane <= kK1TJ && j < ny;
+        const int ib = ti * kK1TI + w * RPW;
+        const unsigned py = (unsigned)A.py, w8 = (unsigned)sizeof(R);
         const glob<R> *h = gmem(A.h[p]), *hu = gmem(A.hu[p]), *hv = gmem(A.hv[p]);
         const glob<double> *zp = gmem(A.z[(p3 + 2) % 3]);
-        if (j < A.ny) {
-            if (i < A.nx) s = wave_speed_cell(A, h, hu, hv, zp, i, j, in);
-            if (i + 4 < A.nx) s = mfmax(s, wave_speed_cell(A, h, hu, hv, zp, i + 4, j, in));
+        auto cell = [&](int row) {
+            row = row < 0 ? 0 : (row >= nx ? nx - 1 : row);
+            return (unsigned)row * py + (unsigned)jc;
+        };
+        // the lagged bed of rows ib-1 .. ib+RPW, depth and momenta of rows ib .. ib+RPW
+        // (hv to ib+RPW-1), rows clamped into the grid
+        double zr[RPW + 2];
+        R hr[RPW + 1], hur[RPW + 1], hvr[RPW];
+#pragma unroll
+        for (int r = 0; r < RPW + 1; r++) {
+            const unsigned c = cell(ib + r);
+            hr[r] = at(h, c * w8);
+            hur[r] = at(hu, c * w8);
+            if (r < RPW) hvr[r] = at(hv, c * w8);
+        }
+#pragma unroll
+        for (int r = 0; r < RPW + 2; r++) zr[r] = at(zp, cell(ib - 1 + r) * 8u);
+        const R hfilm = R(kHfilm);
+        const double odx = A.inv_dx;
+        // velocities of each row (0 below the film depth), one reciprocal per cell
+        R ur[RPW + 1], vr[RPW];
+        bool wet[RPW + 1];
+#pragma unroll
+        for (int r = 0; r < RPW + 1; r++) {
+            wet[r] = hr[r] > hfilm;
+            const R ih = wet[r] ? mrcp(hr[r]) : R(0);
+            ur[r] = wet[r] ? hur[r] * ih : R(0);
+            if (r < RPW) vr[r] = wet[r] ? hvr[r] * ih : R(0);
+        }
+        // the inflow ghost at j = ny (SWEHR.c:1019-1035)
+        const R hin = R(in.h), vin = hin > hfilm ? R(in.hv) * mrcp(hin) : R(0);
+        const double fy = (j != 0 && j != ny - 1) ? 0.5 * odx : odx;
+#pragma unroll
+        for (int r = 0; r < RPW; r++) {
+            const int i = ib + r;
+            // neighbours across the lane shifts (the loads are clamped, so at the domain's
+            // edges they hold the cell itself, as the one-sided slopes need)
+            const double zjp = lane_above(zr[r + 1]), zjm = lane_below(zr[r + 1]);
+            const R hjp = lane_above(hr[r]), vjp = lane_above(vr[r]);
+            if (comp && i < nx) {
+                const double fx = (i != 0 && i != nx - 1) ? 0.5 * odx : odx;
+                const R sx = R(fx * (zr[r + 2] - zr[r])), sy = R(fy * (zjp - zjm));
+                const R g = R(kG) * mrsqrt(sx * sx + sy * sy + R(1));
+                const R hc = hr[r], cl = msqrt(g * hc);
+                const R uc = ur[r], vc = vr[r];
+                // x face i+1/2; at i = nx-1 the ghost mirrors (or copies) the cell
+                R hx = hr[r + 1], ux = ur[r + 1];
+                if (i == nx - 1) {
+                    hx = hc;
+                    ux = wet[r] ? (A.bndry[2] == 1 ? -uc : uc) : R(0);
+                }
+                R sm = face_speed_uc(hc, hx, uc, ux, cl, msqrt(g * hx));
+                // y face j+1/2; at j = ny-1 the inflow ghost or the mirrored cell
+                R hy = hjp, vy = vjp;
+                if (j == ny - 1) {
+                    if (A.inflow) {
+                        hy = hin;
+                        vy = vin;
+                    } else {
+                        hy = hc;
+                        vy = wet[r] ? (A.bndry[3] == 1 ? -vc : vc) : R(0);
+                    }
+                }
+                sm = mfmax(sm, face_speed_uc(hc, hy, vc, vy, cl, msqrt(g * hy)));
+                // faces 1/2: the ghost on the low side, GZ of the first cell (SWEHR.c:1054-1059)
+                if (i == 0) sm = mfmax(sm, face_speed_uc(hc, hc, wet[r] ? (A.bndry[0] == 1 ? -uc : uc) : R(0), uc, cl, cl));
+                if (j == 0) sm = mfmax(sm, face_speed_uc(hc, hc, wet[r] ? (A.bndry[1] == 1 ? -vc : vc) : R(0), vc, cl, cl));
+                s = mfmax(s, sm);
+            }
         }
     }
     // the step kernel's budget partials of the previous step
@@ -417,15 +518,6 @@ struct TileGeom {
     }
 };
 
-// Value held by the lane below (L - 1) by a DPP wave shift; lane 0 keeps its own.
-__device__ __forceinline__ int lane_below_i(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false); }
-__device__ __forceinline__ float lane_below(float v) { return __int_as_float(lane_below_i(__float_as_int(v))); }
-__device__ __forceinline__ double lane_below(double v)
-{
-    const long long b = __double_as_longlong(v);
-    const int lo = lane_below_i((int)(unsigned)b), hi = lane_below_i((int)(b >> 32));
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
 
 // The fused step for one tile: ghosts, face fluxes, sources, update.
 //

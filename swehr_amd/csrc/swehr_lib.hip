@@ -379,7 +379,7 @@ void geometry(swehr_handle *h, int tile)
     h->nblocks = h->tiles_i * h->tiles_j;
     h->blocks_per_xcd = (h->nblocks + 7) / 8;
     const long long n = (long long)h->nx * h->ny;
-    h->k1_blocks = (int)(8 * ((((h->nx + 7) / 8) * ((h->ny + 63) / 64) + 7) / 8));  // 8 x 64 tiles
+    h->k1_blocks = 8 * ((k1_tiles(h->nx, h->ny) + 7) / 8);  // strips of the wave-speed kernel
     h->budget_blocks = (int)std::min<long long>((n + 255) / 256, 1024);
     h->lds = h->precision == SWEHR_FP64 ? tile_lds<double>(tile, h->S) : tile_lds<float>(tile, h->S);
     int slots = 0;

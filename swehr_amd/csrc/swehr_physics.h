@@ -342,14 +342,13 @@ __device__ __forceinline__ R manning2_over_hpow(R h, R n0, const double *exp_tab
     }
 }
 
-// Wave-speed bound of the same face without the fluxes (SWEHR.c:333-376, 404-408).
+// Wave-speed bound of the same face without the fluxes (SWEHR.c:333-376, 404-408),
+// from the normal velocities ul, ur (0 below the film depth) and celerities
+// cl = sqrt(g hl), cr = sqrt(g hr) of the two sides.
 template <typename R>
-__device__ __forceinline__ R face_speed(R hl, R hr, R qnl, R qnr, R g)
+__device__ __forceinline__ R face_speed_uc(R hl, R hr, R ul, R ur, R cl, R cr)
 {
     const R hfilm = R(kHfilm);
-    const R ul = (hl > hfilm) ? qnl * mrcp(hl) : R(0);
-    const R ur = (hr > hfilm) ? qnr * mrcp(hr) : R(0);
-    const R cl = msqrt(g * hl), cr = msqrt(g * hr);
     const R d = ul + R(2) * cl - ur - R(2) * cr;
     const R ustar = R(.5) * (ul + ur) + cl - cr;
     const R cstar = mfabs(d) * R(0.25);
@@ -357,6 +356,16 @@ __device__ __forceinline__ R face_speed(R hl, R hr, R qnl, R qnr, R g)
     if (hl <= hfilm) { sl = ur - R(2) * cr; sm = sl; sr = ur + cr; }
     if (hr <= hfilm) { sl = ul - cl; sr = ul + R(2) * cl; sm = sr; }
     return mfmax(mfmax(R(0), mfabs(sl)), mfmax(mfabs(sr), mfabs(sm)));
+}
+
+// The same from the depths and normal momenta of the two sides and the face's GZ.
+template <typename R>
+__device__ __forceinline__ R face_speed(R hl, R hr, R qnl, R qnr, R g)
+{
+    const R hfilm = R(kHfilm);
+    const R ul = (hl > hfilm) ? qnl * mrcp(hl) : R(0);
+    const R ur = (hr > hfilm) ? qnr * mrcp(hr) : R(0);
+    return face_speed_uc(hl, hr, ul, ur, msqrt(g * hl), msqrt(g * hr));
 }
 
 // Sediment-laden inflow state at j = ny at time t, SWEHR.c:1019-1033 (FP64 always).
