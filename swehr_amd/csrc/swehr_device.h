@@ -176,11 +176,14 @@ __device__ __forceinline__ InflowState ctrl_inflow(const glob<DevCtrl> *C)
     return in;
 }
 
-// Sets the functions of t kept in the control block (one thread).
+// Sets the functions of t kept in the control block (one thread).  The inflow state is
+// read only where the inflow at j = ny is on; without it the pow of inflow_state is
+// not evaluated on the step's critical path.
 template <typename R>
 __device__ __forceinline__ void set_time_functions(glob<DevCtrl> *C, const StepArgs<R> &A, double t)
 {
-    const InflowState in = inflow_state(t);
+    InflowState in = {0.0, 0.0, 0.0};
+    if (A.inflow) in = inflow_state(t);
     C->in_h = in.h;
     C->in_hv = in.hv;
     C->in_c = in.c;
@@ -1048,8 +1051,13 @@ __device__ __forceinline__ void control_body(const StepArgs<R> &A)
     const bool cfl = courant > 1;
     const double tnew = (cfl ? 2 * A.tend : t) + dt;
     const double stageinterval = C->stageinterval;
+    // the fields the end of the step updates, read here with the others (one round trip)
+    const double printinterval = C->printinterval, printint = C->printint;
+    const double rain_sum = C->rain_sum, rain_comp = C->rain_comp;
+    const int pause_on_print = C->pause_on_print, status0 = C->status;
     const int q = (int)((steps + 1) & 1);
     const int cap = A.stage_cap;
+    bool stage_full = false;
     __syncthreads();
     if (tnew > stageinterval) {
         // outlet: deepest cell of row j = 1, first one on ties (SWEHR.c:1596-1604)
@@ -1142,20 +1150,23 @@ __device__ __forceinline__ void control_body(const StepArgs<R> &A)
             }
         }
         if (tid == 0) {
-            if (r >= cap) C->status = 6;  // SWEHR_ERR_STAGE_FULL
+            if (r >= cap) stage_full = true;
             else C->stagecnt = r + 1;
             C->stageinterval = stageinterval + 1;
         }
     }
     if (threadIdx.x == 0) {
-        if (tnew >= C->printinterval) {
-            C->printinterval = C->printinterval + C->printint;
-            if (C->pause_on_print) C->print_due = 1;
+        if (tnew >= printinterval) {
+            C->printinterval = printinterval + printint;
+            if (pause_on_print) C->print_due = 1;
         }
-        if (cfl) C->status = 1;  // SWEHR_ERR_CFL
+        int status = status0;
+        if (stage_full) status = 6;  // SWEHR_ERR_STAGE_FULL
+        if (cfl) status = 1;         // SWEHR_ERR_CFL
         // dt = 0 (infinite wave speed) would never reach tend; the reference loops forever
-        if (!(dt > 0) && tnew < A.tend) C->status = 2;  // SWEHR_ERR_NONFINITE
-        double rs = C->rain_sum, rc = C->rain_comp;
+        if (!(dt > 0) && tnew < A.tend) status = 2;  // SWEHR_ERR_NONFINITE
+        if (status != status0) C->status = status;
+        double rs = rain_sum, rc = rain_comp;
         neumaier(rs, rc, dt * Rr * ((double)A.nx * A.ny) * A.dx * A.dx);
         C->rain_sum = rs;
         C->rain_comp = rc;
@@ -1166,7 +1177,7 @@ __device__ __forceinline__ void control_body(const StepArgs<R> &A)
         C->courant = courant;
         C->rain = Rr;
         C->steps = steps + 1;
-        if (!(tnew < A.tend) || C->status != 0) C->done = 1;
+        if (!(tnew < A.tend) || status != 0) C->done = 1;
     }
     __syncthreads();  // every wave has read the slots
     if (threadIdx.x < 64) C->smax_slots[threadIdx.x] = 0ull;
