@@ -13,15 +13,20 @@
 // are not stored: they are exact functions of (h, ch_k, hu, hv) (SWEHR.c:1516-1531,
 // 1563-1564, 1578-1579) and are recomputed where needed.
 //
-// One step = three launches, all guarded by the per-instance device control block so
-// that a captured graph of many steps stops by itself at t >= tend, at a print pause
-// or at a step limit:
+// A step's work, guarded by the per-instance device control block so that a captured
+// graph of many steps stops by itself at t >= tend, at a print pause or at a step limit:
 //   K1 wave speeds  max |wave speed| over every face (SWEHR.c:1161, 1228) -> atomicMax
 //                   on the bit pattern of a non-negative double
 //   K2 step         boundary ghosts, fluxes, sources, dt, update (SWEHR.c:816-1589) on
 //                   LDS-staged tiles; every face flux is computed once per tile and the
-//                   per-class work streams through registers
-//   K3 control      t += dt, outlet and probe records, print pause (SWEHR.c:1591-1682)
+//                   per-class work streams through registers; one thread also decides
+//                   the end of the step (t += dt, print pause, stop: SWEHR.c:1408-1413,
+//                   1591-1592, 1679-1682) into the control block's `next`
+//   commit          outlet and probe records (SWEHR.c:1593-1676) and the decided end
+//                   into the control block: in a single instance's step graph by block
+//                   0 of the next step's K1, and by K3 (the control kernel) at the end
+//                   of the graph; in the ensemble modes and for steps launched one by
+//                   one by K3 after every step
 // The bodies are device functions; the single-instance kernels and the ensemble
 // kernels (one launch over the tiles of many instances) call the same code.
 #pragma once
@@ -40,6 +45,18 @@ constexpr int kStageMax = 30000;
 // per-tile budget terms accumulated across steps (sum and Neumaier compensation)
 enum BudgetTerm { B_EXCH = 0, B_CLAMP = 1, B_DET = 2, B_BWATER = 3, B_BSED = 4, NB_TERMS = 5 };
 
+// The end of a step as the control block will hold it (its scalar part): formed by the
+// step kernel as soon as the step's dt is known, committed to the control block once
+// every cell of the step is stored, together with the stage records that read them.
+struct StepNext {
+    double t, dt, courant, rain, t_prev;
+    double stageinterval, printinterval, rain_sum, rain_comp;
+    double in_h, in_hv, in_c, rain_now;
+    long long steps;
+    int stagecnt, status, done, print_due;
+    int stage_due, stage_r, active, pad;
+};
+
 struct alignas(64) DevCtrl {
     double t, dt, courant, rain;
     double t_prev;  // time at the start of the last step (the CFL message prints it)
@@ -50,10 +67,15 @@ struct alignas(64) DevCtrl {
     double in_h, in_hv, in_c, rain_now;
     long long steps, step_limit;
     int stagecnt, status, done, print_due;
-    int pause_on_print, pad[3];
-    // max |wave speed| of the step as bit patterns of non-negative doubles, spread over
-    // 64 slots so that the per-block atomicMax of K1 does not serialise on one address
-    unsigned long long smax_slots[64];
+    // pending: `next` holds the decision of a step whose kernel ran since the last
+    // standalone commit (the control kernel clears it; a graph always ends with one)
+    int pause_on_print, pending, pad[2];
+    StepNext next;
+    // max |wave speed| of a step as bit patterns of non-negative doubles, spread over 64
+    // slots so that the per-block atomicMax of K1 does not serialise on one address;
+    // one set per step parity, so that K1 of a step can fill its set while the commit
+    // of the previous step clears the other
+    unsigned long long smax_slots[2][64];
 };
 
 template <typename R>
@@ -160,11 +182,11 @@ __device__ __forceinline__ bool step_active(const glob<DevCtrl> *C)
     return !C->done && !C->print_due && C->steps < C->step_limit;
 }
 
-// Max over the 64 wave-speed slots; every lane of the calling wave gets it (all 64
-// lanes must be active).
-__device__ __forceinline__ double ctrl_smax(const glob<DevCtrl> *C)
+// Max over the 64 wave-speed slots of the step with count `steps`; every lane of the
+// calling wave gets it (all 64 lanes must be active).
+__device__ __forceinline__ double ctrl_smax(const glob<DevCtrl> *C, long long steps)
 {
-    return wave_max(__longlong_as_double((long long)C->smax_slots[threadIdx.x & 63]));
+    return wave_max(__longlong_as_double((long long)C->smax_slots[steps & 1][threadIdx.x & 63]));
 }
 
 __device__ __forceinline__ InflowState ctrl_inflow(const glob<DevCtrl> *C)
@@ -174,6 +196,33 @@ __device__ __forceinline__ InflowState ctrl_inflow(const glob<DevCtrl> *C)
     in.hv = C->in_hv;
     in.c = C->in_c;
     return in;
+}
+
+// What the wave-speed pass of the next step needs of the control block: whether that
+// step runs, its count (the parity of the buffers) and the inflow state at its time.
+// While a decision is pending (the previous step's commit runs in the same launch, in
+// block 0), they come from the decision, which no block of that launch writes.
+struct K1View {
+    bool active;
+    long long steps;
+    InflowState in;
+};
+__device__ __forceinline__ K1View k1_view(const glob<DevCtrl> *C)
+{
+    // both sets are read at once and selected (one round trip)
+    const glob<StepNext> *N = &C->next;
+    const bool pend = C->pending != 0;
+    const bool act_c = step_active(C), act_n = N->active != 0;
+    const long long steps_c = C->steps, steps_n = N->steps;
+    const InflowState in_c = ctrl_inflow(C);
+    const double nh = N->in_h, nhv = N->in_hv, nc = N->in_c;
+    K1View v;
+    v.active = pend ? act_n : act_c;
+    v.steps = pend ? steps_n : steps_c;
+    v.in.h = pend ? nh : in_c.h;
+    v.in.hv = pend ? nhv : in_c.hv;
+    v.in.c = pend ? nc : in_c.c;
+    return v;
 }
 
 // Sets the functions of t kept in the control block (one thread).  The inflow state is
@@ -243,6 +292,222 @@ __device__ __forceinline__ void fold_budget(glob<double> *bud, long long nterms,
     p[0] = s;
     p[1] = c;
     *pend = 0.0;
+}
+
+
+// The scalar end of the current step (SWEHR.c:1408-1413 dt and the CFL stop,
+// 1591-1592 the stage clock, 1679-1682 the print interval, the rain budget, and the
+// functions of the new time), from the control block and the step's max wave speed smax,
+// written to C->next.  One thread, while the step's kernels run; nothing of that launch
+// reads C->next.
+template <typename R>
+__device__ __forceinline__ void decide_step(glob<DevCtrl> *C, const StepArgs<R> &A, double smax)
+{
+    const long long steps = C->steps;
+    const double t = C->t;
+    const double dt = step_dt(smax, A.cstable, A.dx, A.maxstep, t, A.tend);
+    const double courant = (smax)*dt * (1.0 / A.dx);
+    const double Rr = C->rain_now;
+    const bool cfl = courant > 1;
+    const double tnew = (cfl ? 2 * A.tend : t) + dt;
+    const double stageinterval = C->stageinterval;
+    const int r = C->stagecnt, status0 = C->status;
+    const bool stage_due = tnew > stageinterval, stage_full = stage_due && r >= A.stage_cap;
+    double printinterval = C->printinterval;
+    int print_due = 0;
+    if (tnew >= printinterval) {
+        printinterval = printinterval + C->printint;
+        if (C->pause_on_print) print_due = 1;
+    }
+    int status = status0;
+    if (stage_full) status = 6;  // SWEHR_ERR_STAGE_FULL
+    if (cfl) status = 1;         // SWEHR_ERR_CFL
+    // dt = 0 (infinite wave speed) would never reach tend; the reference loops forever
+    if (!(dt > 0) && tnew < A.tend) status = 2;  // SWEHR_ERR_NONFINITE
+    double rs = C->rain_sum, rc = C->rain_comp;
+    neumaier(rs, rc, dt * Rr * ((double)A.nx * A.ny) * A.dx * A.dx);
+    const int done = (!(tnew < A.tend) || status != 0) ? 1 : 0;
+    // the inflow state is read only where the inflow at j = ny is on; without it the pow
+    // of inflow_state is not evaluated
+    InflowState in = {0.0, 0.0, 0.0};
+    if (A.inflow) in = inflow_state(tnew);
+    glob<StepNext> *N = &C->next;
+    N->t = tnew;
+    N->dt = dt;
+    N->courant = courant;
+    N->rain = Rr;
+    N->t_prev = t;
+    N->stageinterval = stage_due ? stageinterval + 1 : stageinterval;
+    N->printinterval = printinterval;
+    N->rain_sum = rs;
+    N->rain_comp = rc;
+    N->in_h = in.h;
+    N->in_hv = in.hv;
+    N->in_c = in.c;
+    N->rain_now = rain_rate(A.rain, A.rnum, A.rint, tnew);
+    N->steps = steps + 1;
+    N->stagecnt = stage_due && !stage_full ? r + 1 : r;
+    N->status = status;
+    N->done = done;
+    N->print_due = print_due;
+    N->stage_due = stage_due ? 1 : 0;
+    N->stage_r = r;
+    N->active = (!done && !print_due && steps + 1 < C->step_limit) ? 1 : 0;
+    C->pending = 1;
+}
+
+// Derived quantities of a stored cell (SWEHR.c:1516-1531, 1563-1564).
+template <typename R>
+__device__ __forceinline__ void derived(const StepArgs<R> &A, int p, long long c, double &h, double &v1, double &v2,
+                                        double &ts)
+{
+    const R hh = gmem(A.h[p])[c];
+    h = (double)hh;
+    v1 = v2 = ts = 0;
+    if (hh > R(kHfilm)) {
+        v1 = (double)(gmem(A.hu[p])[c] / hh);
+        v2 = (double)(gmem(A.hv[p])[c] / hh);
+        R s = R(0);
+        const glob<R> *ch = gmem(A.ch[p]);
+        for (int k = 0; k < A.S; k++) s += ch[k * A.plane + c] / hh / R(kRhos);
+        ts = (double)s;
+    }
+}
+
+// Commits the decision in C->next of the step just completed: its stage records
+// (SWEHR.c:1593-1676, which read the step's new state), the control block's fields, and
+// clears that step's wave-speed slots.  Run by all threads of one block (blocks of up to
+// 1024 threads).
+template <typename R>
+__device__ __forceinline__ void commit_step(const StepArgs<R> &A, glob<DevCtrl> *C)
+{
+    __shared__ double s_best[16], s_q2[16];  // one per wave
+    __shared__ int s_idx[16];
+    StepNext N;
+    {
+        const glob<StepNext> *G = &C->next;
+        N.t = G->t;
+        N.steps = G->steps;
+        N.stage_due = G->stage_due;
+        N.stage_r = G->stage_r;
+    }
+    const double tnew = N.t;
+    const int q = (int)(N.steps & 1);
+    const int cap = A.stage_cap;
+    const int r = N.stage_r;
+    if (N.stage_due) {
+        // outlet: deepest cell of row j = 1, first one on ties (SWEHR.c:1596-1604)
+        double best = 0;
+        int idx = 0x7fffffff;
+#pragma unroll 8
+        for (int i = threadIdx.x; i < A.nx; i += blockDim.x) {
+            const double hv = (double)gmem(A.h[q])[(long long)i * A.py];
+            if (hv > best) { best = hv; idx = i; }
+        }
+        // block arg-max: larger depth wins, the lower index on equal depths
+        for (int o = 32; o > 0; o >>= 1) {
+            const double ob = __shfl_xor(best, o, 64);
+            const int oi = __shfl_xor(idx, o, 64);
+            if (ob > best || (ob == best && oi < idx)) { best = ob; idx = oi; }
+        }
+        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+        if (lane == 0) {
+            s_best[wid] = best;
+            s_idx[wid] = idx;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 1; w < (int)(blockDim.x >> 6); w++)
+                if (s_best[w] > best || (s_best[w] == best && s_idx[w] < idx)) { best = s_best[w]; idx = s_idx[w]; }
+            s_idx[0] = best > 0 ? idx : 0;
+        }
+        __syncthreads();
+        const int outlet = s_idx[0];
+        // outlet discharge of this step: the columns' outflows summed in a fixed order
+        {
+            double q = 0, qs = 0;
+            const glob<double> *oq = gmem(A.outq), *os = gmem(A.outs);
+            if (threadIdx.x < 256)  // stride 256 whatever the block size: the same order everywhere
+#pragma unroll 8
+                for (int i = threadIdx.x; i < A.nx; i += 256) {
+                    q += oq[i];
+                    qs += os[i];
+                }
+            q = wave_sum(q);
+            qs = wave_sum(qs);
+            __syncthreads();  // s_best is reused
+            if (lane == 0) {
+                s_best[wid] = q;
+                s_q2[wid] = qs;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0 && r < cap) {
+                double tq = 0, tqs = 0;
+                for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+                    tq += s_best[w];
+                    tqs += s_q2[w];
+                }
+                gmem(A.discharge)[r] = tq;
+                gmem(A.discharge)[cap + r] = tqs;
+            }
+        }
+        // one thread per stage column group: outlet state, its classes, the six probes
+        // at (i = 2, j = 1, 10, ..., 50) (SWEHR.c:1605-1644)
+        const int tid = threadIdx.x;
+        if (r < cap) {
+            glob<double> *st = gmem(A.stage);
+            const long long co = (long long)outlet * A.py;
+            if (tid == 0) {
+                double h, v1, v2, ts;
+                derived(A, q, co, h, v1, v2, ts);
+                st[0 * cap + r] = tnew;
+                st[1 * cap + r] = h;
+                st[2 * cap + r] = sqrt(v1 * v1 + v2 * v2);
+                st[3 * cap + r] = ts;
+            } else if (tid >= 1 && tid <= 10) {
+                const int k = tid - 1;
+                st[(4 + k) * cap + r] = k < A.S ? (double)gmem(A.M)[k * A.plane + co] : 0.0;
+                st[(14 + k) * cap + r] = k < A.S ? (double)gmem(A.ch[q])[k * A.plane + co] : 0.0;
+            } else if (tid >= 11 && tid <= 16) {
+                const int n = tid - 11;
+                const int yp = n == 0 ? 1 : 10 * n;
+                double d = 0, sp = 0, tt = 0;
+                if (A.nx >= 2 && yp <= A.ny) {
+                    double hh, w1, w2;
+                    derived(A, q, (long long)1 * A.py + (yp - 1), hh, w1, w2, tt);
+                    d = hh;
+                    sp = sqrt(w1 * w1 + w2 * w2);
+                }
+                glob<double> *sf = gmem(A.saveflow);
+                sf[(3 * n + 0) * cap + r] = d;
+                sf[(3 * n + 1) * cap + r] = sp;
+                sf[(3 * n + 2) * cap + r] = tt;
+            }
+        }
+    }
+    __syncthreads();  // every thread has read the decision
+    if (threadIdx.x == 0) {
+        const glob<StepNext> *G = &C->next;
+        C->t = G->t;
+        C->dt = G->dt;
+        C->courant = G->courant;
+        C->rain = G->rain;
+        C->t_prev = G->t_prev;
+        C->stageinterval = G->stageinterval;
+        C->printinterval = G->printinterval;
+        C->rain_sum = G->rain_sum;
+        C->rain_comp = G->rain_comp;
+        C->in_h = G->in_h;
+        C->in_hv = G->in_hv;
+        C->in_c = G->in_c;
+        C->rain_now = G->rain_now;
+        C->stagecnt = G->stagecnt;
+        if (G->status != C->status) C->status = G->status;
+        if (G->print_due) C->print_due = 1;
+        if (G->done) C->done = 1;
+        C->steps = G->steps;
+    }
+    if (threadIdx.x < 64) C->smax_slots[(N.steps - 1) & 1][threadIdx.x] = 0ull;
 }
 
 }  // namespace swehr
@@ -342,16 +607,20 @@ __global__ __launch_bounds__(256) void wave_speed_kernel(StepArgs<R> A)
 {
     constexpr int RPW = kK1Rows;
     glob<DevCtrl> *C = gmem(A.ctrl);
-    if (!step_active(C)) return;
-    const long long steps = C->steps;
+    const int b = blockIdx.x;
+    const K1View view = k1_view(C);
+    // block 0 also commits the previous step (its stage records read that step's stored
+    // cells, which the launch boundary has made visible)
+    if (!view.active && b != 0) return;
+    const long long steps = view.steps;
     const int p = (int)(steps & 1), p3 = (int)(steps % 3);
-    const InflowState in = ctrl_inflow(C);
+    const InflowState in = view.in;
     const int nx = A.nx, ny = A.ny;
     const int tiles_j = (ny + kK1TJ - 1) / kK1TJ, ntiles = k1_tiles(nx, ny);
-    const int b = blockIdx.x, per = (int)(gridDim.x >> 3);
+    const int per = (int)(gridDim.x >> 3);
     const int tile = (b & 7) * per + (b >> 3);
     R s = R(0);
-    if (tile < ntiles) {
+    if (view.active && tile < ntiles) {
         const int ti = tile / tiles_j, tj = tile - ti * tiles_j;
         const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
         const int j = tj * kK1TJ - 1 + lane;  // this lane's column (-1 or >= ny at the edges)
@@ -432,11 +701,20 @@ __global__ __launch_bounds__(256) void wave_speed_kernel(StepArgs<R> A)
             }
         }
     }
-    // the step kernel's budget partials of the previous step
-    const long long nterms = (long long)A.bud_slots * NB_TERMS;
-    for (long long g = (long long)b * blockDim.x + threadIdx.x; g < nterms; g += (long long)gridDim.x * blockDim.x)
-        fold_budget(gmem(A.bud), nterms, g);
-    block_max_to((unsigned long long *)C->smax_slots, b, (double)s);
+    if (view.active) {
+        // the step kernel's budget partials of the previous step
+        const long long nterms = (long long)A.bud_slots * NB_TERMS;
+        for (long long g = (long long)b * blockDim.x + threadIdx.x; g < nterms;
+             g += (long long)gridDim.x * blockDim.x)
+            fold_budget(gmem(A.bud), nterms, g);
+        block_max_to((unsigned long long *)C->smax_slots[steps & 1], b, (double)s);
+    }
+    if (b == 0) {
+        __shared__ int s_commit;
+        if (threadIdx.x == 0) s_commit = C->pending && C->steps + 1 == C->next.steps;
+        __syncthreads();
+        if (s_commit) commit_step(A, C);
+    }
 }
 
 // Wave speeds of one TI x TJ tile (ensemble path: one block per tile).
@@ -444,10 +722,11 @@ template <typename R, int TI, int TJ>
 __device__ __forceinline__ void wave_speed_tile(const StepArgs<R> &A, int tile)
 {
     glob<DevCtrl> *C = gmem(A.ctrl);
-    if (!step_active(C)) return;
-    const long long steps = C->steps;
+    const K1View view = k1_view(C);
+    if (!view.active) return;
+    const long long steps = view.steps;
     const int p = (int)(steps & 1), p3 = (int)(steps % 3);
-    const InflowState in = ctrl_inflow(C);
+    const InflowState in = view.in;
     const int ti = tile / A.tiles_j, tj = tile - (tile / A.tiles_j) * A.tiles_j;
     const int il = threadIdx.x / TJ, jl = threadIdx.x - (threadIdx.x / TJ) * TJ;
     const int i = ti * TI + il, j = tj * TJ + jl;
@@ -458,7 +737,7 @@ __device__ __forceinline__ void wave_speed_tile(const StepArgs<R> &A, int tile)
     if ((int)threadIdx.x < (TI * TJ / 64) * NB_TERMS)
         fold_budget(gmem(A.bud), (long long)A.bud_slots * NB_TERMS,
                     (long long)tile * (TI * TJ / 64) * NB_TERMS + threadIdx.x);
-    block_max_to((unsigned long long *)C->smax_slots, tile, (double)s);
+    block_max_to((unsigned long long *)C->smax_slots[steps & 1], tile, (double)s);
 }
 
 // --------------------------------------------------------------------------- K2
@@ -549,9 +828,12 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
 
     const long long steps = C->steps;
     const double t = C->t;
-    const double smax = ctrl_smax(C);
+    const double smax = ctrl_smax(C, steps);
     const double dtd = step_dt(smax, A.cstable, A.dx, A.maxstep, t, A.tend);
     const double Rd = C->rain_now;
+    // the end of this step, decided by one thread of the instance's first tile while the
+    // step runs; it is committed once every tile has stored its cells
+    if (tile == 0 && threadIdx.x == 0) decide_step(C, A, smax);
     const int p = (int)(steps & 1), q = p ^ 1, p3 = (int)(steps % 3);
     const int nx = A.nx, ny = A.ny, py = A.py, S = A.S;
     const long long plane = A.plane;
@@ -1016,171 +1298,21 @@ __global__ __launch_bounds__(TI *TJ, (kStepWaves<R, SPLIT>)) void step_kernel(St
 
 // --------------------------------------------------------------------------- K3
 
-// Derived quantities of a stored cell (SWEHR.c:1516-1531, 1563-1564).
-template <typename R>
-__device__ __forceinline__ void derived(const StepArgs<R> &A, int p, long long c, double &h, double &v1, double &v2,
-                                        double &ts)
-{
-    const R hh = gmem(A.h[p])[c];
-    h = (double)hh;
-    v1 = v2 = ts = 0;
-    if (hh > R(kHfilm)) {
-        v1 = (double)(gmem(A.hu[p])[c] / hh);
-        v2 = (double)(gmem(A.hv[p])[c] / hh);
-        R s = R(0);
-        const glob<R> *ch = gmem(A.ch[p]);
-        for (int k = 0; k < A.S; k++) s += ch[k * A.plane + c] / hh / R(kRhos);
-        ts = (double)s;
-    }
-}
-
-// End-of-step bookkeeping of one instance, run by one block.
+// The standalone commit: the control kernel after a step's kernels (in the step graph
+// after the last step; every step in the ensemble modes and where steps are launched
+// one by one).  Commits the pending decision unless the wave-speed pass of a following
+// step already did, and clears the pending mark.
 template <typename R>
 __device__ __forceinline__ void control_body(const StepArgs<R> &A)
 {
     glob<DevCtrl> *C = gmem(A.ctrl);
-    if (!step_active(C)) return;
-    __shared__ double s_best[16], s_q2[16];  // one per wave (blocks of up to 1024 threads)
-    __shared__ int s_idx[16];
-    const long long steps = C->steps;
-    const double t = C->t;
-    const double smax = ctrl_smax(C);
-    const double dt = step_dt(smax, A.cstable, A.dx, A.maxstep, t, A.tend);
-    const double courant = (smax)*dt * (1.0 / A.dx);
-    const double Rr = C->rain_now;
-    const bool cfl = courant > 1;
-    const double tnew = (cfl ? 2 * A.tend : t) + dt;
-    const double stageinterval = C->stageinterval;
-    // the fields the end of the step updates, read here with the others (one round trip)
-    const double printinterval = C->printinterval, printint = C->printint;
-    const double rain_sum = C->rain_sum, rain_comp = C->rain_comp;
-    const int pause_on_print = C->pause_on_print, status0 = C->status;
-    const int q = (int)((steps + 1) & 1);
-    const int cap = A.stage_cap;
-    bool stage_full = false;
+    __shared__ int s_todo;
+    if (threadIdx.x == 0) s_todo = C->pending ? (C->steps + 1 == C->next.steps ? 2 : 1) : 0;
     __syncthreads();
-    if (tnew > stageinterval) {
-        // outlet: deepest cell of row j = 1, first one on ties (SWEHR.c:1596-1604)
-        double best = 0;
-        int idx = 0x7fffffff;
-#pragma unroll 8
-        for (int i = threadIdx.x; i < A.nx; i += blockDim.x) {
-            const double hv = (double)gmem(A.h[q])[(long long)i * A.py];
-            if (hv > best) { best = hv; idx = i; }
-        }
-        // block arg-max: larger depth wins, the lower index on equal depths
-        for (int o = 32; o > 0; o >>= 1) {
-            const double ob = __shfl_xor(best, o, 64);
-            const int oi = __shfl_xor(idx, o, 64);
-            if (ob > best || (ob == best && oi < idx)) { best = ob; idx = oi; }
-        }
-        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-        if (lane == 0) {
-            s_best[wid] = best;
-            s_idx[wid] = idx;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            for (int w = 1; w < (int)(blockDim.x >> 6); w++)
-                if (s_best[w] > best || (s_best[w] == best && s_idx[w] < idx)) { best = s_best[w]; idx = s_idx[w]; }
-            s_idx[0] = best > 0 ? idx : 0;
-        }
-        __syncthreads();
-        const int outlet = s_idx[0];
-        const int r = C->stagecnt;
-        // outlet discharge of this step: the columns' outflows summed in a fixed order
-        {
-            double q = 0, qs = 0;
-            const glob<double> *oq = gmem(A.outq), *os = gmem(A.outs);
-            if (threadIdx.x < 256)  // stride 256 whatever the block size: the same order everywhere
-#pragma unroll 8
-                for (int i = threadIdx.x; i < A.nx; i += 256) {
-                    q += oq[i];
-                    qs += os[i];
-                }
-            q = wave_sum(q);
-            qs = wave_sum(qs);
-            __syncthreads();  // s_best is reused
-            if (lane == 0) {
-                s_best[wid] = q;
-                s_q2[wid] = qs;
-            }
-            __syncthreads();
-            if (threadIdx.x == 0 && r < cap) {
-                double tq = 0, tqs = 0;
-                for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
-                    tq += s_best[w];
-                    tqs += s_q2[w];
-                }
-                gmem(A.discharge)[r] = tq;
-                gmem(A.discharge)[cap + r] = tqs;
-            }
-        }
-        // one thread per stage column group: outlet state, its classes, the six probes
-        // at (i = 2, j = 1, 10, ..., 50) (SWEHR.c:1605-1644)
-        const int tid = threadIdx.x;
-        if (r < cap) {
-            glob<double> *st = gmem(A.stage);
-            const long long co = (long long)outlet * A.py;
-            if (tid == 0) {
-                double h, v1, v2, ts;
-                derived(A, q, co, h, v1, v2, ts);
-                st[0 * cap + r] = tnew;
-                st[1 * cap + r] = h;
-                st[2 * cap + r] = sqrt(v1 * v1 + v2 * v2);
-                st[3 * cap + r] = ts;
-            } else if (tid >= 1 && tid <= 10) {
-                const int k = tid - 1;
-                st[(4 + k) * cap + r] = k < A.S ? (double)gmem(A.M)[k * A.plane + co] : 0.0;
-                st[(14 + k) * cap + r] = k < A.S ? (double)gmem(A.ch[q])[k * A.plane + co] : 0.0;
-            } else if (tid >= 11 && tid <= 16) {
-                const int n = tid - 11;
-                const int yp = n == 0 ? 1 : 10 * n;
-                double d = 0, sp = 0, tt = 0;
-                if (A.nx >= 2 && yp <= A.ny) {
-                    double hh, w1, w2;
-                    derived(A, q, (long long)1 * A.py + (yp - 1), hh, w1, w2, tt);
-                    d = hh;
-                    sp = sqrt(w1 * w1 + w2 * w2);
-                }
-                glob<double> *sf = gmem(A.saveflow);
-                sf[(3 * n + 0) * cap + r] = d;
-                sf[(3 * n + 1) * cap + r] = sp;
-                sf[(3 * n + 2) * cap + r] = tt;
-            }
-        }
-        if (tid == 0) {
-            if (r >= cap) stage_full = true;
-            else C->stagecnt = r + 1;
-            C->stageinterval = stageinterval + 1;
-        }
-    }
-    if (threadIdx.x == 0) {
-        if (tnew >= printinterval) {
-            C->printinterval = printinterval + printint;
-            if (pause_on_print) C->print_due = 1;
-        }
-        int status = status0;
-        if (stage_full) status = 6;  // SWEHR_ERR_STAGE_FULL
-        if (cfl) status = 1;         // SWEHR_ERR_CFL
-        // dt = 0 (infinite wave speed) would never reach tend; the reference loops forever
-        if (!(dt > 0) && tnew < A.tend) status = 2;  // SWEHR_ERR_NONFINITE
-        if (status != status0) C->status = status;
-        double rs = rain_sum, rc = rain_comp;
-        neumaier(rs, rc, dt * Rr * ((double)A.nx * A.ny) * A.dx * A.dx);
-        C->rain_sum = rs;
-        C->rain_comp = rc;
-        C->t_prev = t;
-        C->t = tnew;
-        set_time_functions(C, A, tnew);
-        C->dt = dt;
-        C->courant = courant;
-        C->rain = Rr;
-        C->steps = steps + 1;
-        if (!(tnew < A.tend) || status != 0) C->done = 1;
-    }
-    __syncthreads();  // every wave has read the slots
-    if (threadIdx.x < 64) C->smax_slots[threadIdx.x] = 0ull;
+    const int todo = s_todo;
+    if (todo == 0) return;
+    if (todo == 2) commit_step(A, C);
+    if (threadIdx.x == 0) C->pending = 0;
 }
 
 template <typename R>

@@ -460,11 +460,17 @@ int upload_raster(swehr_handle *h, R *dst, const double *src)
     return SWEHR_OK;
 }
 
+// The steps of one graph: the wave-speed pass and the step kernel per step; the
+// wave-speed pass of a step also commits the previous step (its block 0), and the graph
+// ends with the control kernel, which commits the last step, so that the control block
+// the host reads after the graph is complete.
 template <typename R>
-int enqueue_step(swehr_handle *h, const StepArgs<R> &A)
+int enqueue_steps(swehr_handle *h, const StepArgs<R> &A, int steps)
 {
-    hipLaunchKernelGGL((wave_speed_kernel<R>), dim3(h->k1_blocks), dim3(256), 0, h->stream, A);
-    launch_step<R>(h->tile, 8 * h->blocks_per_xcd, h->lds, h->stream, A);
+    for (int s = 0; s < steps; s++) {
+        hipLaunchKernelGGL((wave_speed_kernel<R>), dim3(h->k1_blocks), dim3(256), 0, h->stream, A);
+        launch_step<R>(h->tile, 8 * h->blocks_per_xcd, h->lds, h->stream, A);
+    }
     hipLaunchKernelGGL((control_kernel<R>), dim3(1), dim3(256), 0, h->stream, A);
     HIPCHK(hipGetLastError());
     return SWEHR_OK;
@@ -481,9 +487,8 @@ int build_graph(swehr_handle *h)
         h->graph = nullptr;
     }
     HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    int rc = SWEHR_OK;
-    for (int s = 0; s < h->graph_steps && rc == SWEHR_OK; s++)
-        rc = h->precision == SWEHR_FP64 ? enqueue_step(h, h->a64) : enqueue_step(h, h->a32);
+    const int rc =
+        h->precision == SWEHR_FP64 ? enqueue_steps(h, h->a64, h->graph_steps) : enqueue_steps(h, h->a32, h->graph_steps);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(h->stream, &g);
     if (rc != SWEHR_OK) return rc;

@@ -272,6 +272,44 @@ def test_every_tile_shape_gives_the_same_bits(precision):
     assert ran == [1, 2, 3, 4]
 
 
+@pytest.mark.parametrize("precision", [_abi.FP64, _abi.FP32], ids=["fp64", "fp32"])
+def test_step_commit_in_next_wave_speed_pass_equals_commit_per_step(precision):
+    """In a step graph the wave-speed pass of each step commits the previous step's end
+    (time, stage records, print pause, stop) and the graph ends with the control kernel;
+    a graph of one step commits every step in its control kernel.  Runs cut at step
+    counts that end inside a graph, with print pauses, must give the same bits both ways:
+    every field, the stage series, the outlet discharge, the step count and time."""
+    from parity_util import many_classes_case
+
+    case = many_classes_case(5, 40, 66, seed=11)
+    cuts = [1, 15, 16, 17, 31, 5, 40, 25]
+    got = []
+    for spg in (16, 1):
+        s = sv.Solver(case, precision=precision, steps_per_graph=spg, pause_on_print=True)
+        pauses = 0
+        for n in cuts:
+            left = n
+            while left > 0:
+                before = s.info()["steps"]
+                info = s.advance(left)
+                left -= info["steps"] - before
+                if info["print_due"]:
+                    pauses += 1
+                    s.ack_print()
+                if info["done"]:
+                    break
+        st = s.state()
+        got.append((st, s.stage(), s.discharge(), s.info(), pauses))
+        s.close()
+    (a, sa, qa, ia, pa), (b, sb, qb, ib, pb) = got
+    print("steps", ia["steps"], "t", ia["t"], "print pauses", pa, pb)
+    assert ia["steps"] == sum(cuts) and ia["steps"] == ib["steps"] and ia["t"] == ib["t"] and pa == pb > 0
+    for f in a:
+        np.testing.assert_array_equal(a[f], b[f], err_msg=f)
+    for x, y in zip(sa + qa, sb + qb):
+        np.testing.assert_array_equal(x, y)
+
+
 @pytest.mark.parametrize("precision,tol", [(_abi.FP64, RTOL_FP64_SHORT), (_abi.FP32, 1e-4)], ids=["fp64", "fp32"])
 def test_channel_network_case_matches_oracle(precision, tol):
     """The inputs of the 4096^2 precision-form case at 72 x 56 (ragged against the tiles):
