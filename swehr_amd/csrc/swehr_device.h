@@ -752,8 +752,8 @@ constexpr int kStepWaves = sizeof(R) == 4 ? (SPLIT ? 4 : 5) : 3;
 // 2 on.
 constexpr int kFp32MOwn = 1;
 // FP64 two-chunk variant: classes whose M_k is loaded up front, and whose ch_k stays in
-// registers (8 and 7: the most its 168-VGPR budget holds without a spill)
-constexpr int kSplitMOwn = 8;
+// registers (7 and 7: with 8 M_k up front the 168-VGPR budget spills 12 bytes per lane)
+constexpr int kSplitMOwn = 7;
 constexpr int kSplitOwn = 7;
 // FP64: classes whose ch_k and M_k of the own cell stay in registers from the halo
 // staging on (the rest are read one class ahead in the class loop, where the first of
