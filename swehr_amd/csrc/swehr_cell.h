@@ -47,6 +47,9 @@ struct Cell {
     R totm, depn, chtemp, mv_o, md_o, totm_o;
     double zn_c, sb_out, sq_out;
     double bud[NB_TERMS];
+    // FP32: the class loop's bed, deposition and exchange increments summed in FP32 and
+    // added to zn_c and the budget terms once per cell (cell_end)
+    R dz, bdet, bexch;
     unsigned rbits;
 };
 
@@ -229,6 +232,7 @@ __device__ __forceinline__ void cell_begin(Cell<R> &X, const StepArgs<R> &A, con
     X.fy0r = in.fyr[0];
     X.hdt = R(0.5) * dt;
     X.zn_c = in.zn;
+    X.dz = X.bdet = X.bexch = R(0);
     X.totm = R(0);
     X.depn = R(0);
     X.chtemp = R(0);
@@ -239,18 +243,20 @@ __device__ __forceinline__ void cell_begin(Cell<R> &X, const StepArgs<R> &A, con
     X.md_o = in.md_o;
     X.totm_o = in.totm_o;
     // random bits of the unbiased roundings (FP32): a hash of the cell and the step,
-    // then a linear congruential sequence (the Numerical Recipes multiplier and
-    // increment, mod 2^32; the high bits are used) over the values the cell stores,
-    // so results are reproducible and do not depend on the tile shape or on the
-    // ensemble
+    // then a Weyl sequence over the values the cell stores (cell_next_bits), so
+    // results are reproducible and do not depend on the tile shape or on the ensemble
     X.rbits = sizeof(R) == sizeof(float) ? mix32(c ^ mix32((unsigned)steps)) : 0u;
     X.edge = gi == 0 || gi == nx - 1 || gj == 0 || gj == ny - 1;
 }
 
+// The next 32 random bits of the cell's unbiased roundings: a Weyl sequence (the
+// golden-ratio increment 0x9e3779b9, odd, so it visits every 32-bit value) from the
+// hash of the cell and the step; each draw is uniform because its start is, which is
+// all an unbiased rounding needs, at one integer add per draw.
 template <typename R>
 __device__ __forceinline__ unsigned cell_next_bits(Cell<R> &X)
 {
-    X.rbits = X.rbits * 1664525u + 1013904223u;
+    X.rbits += 0x9e3779b9u;
     return X.rbits;
 }
 
@@ -281,7 +287,7 @@ __device__ __forceinline__ void cell_class(Cell<R> &X, const StepArgs<R> &A, int
     // sources of class k from the state before this update (SWEHR.c:1322-1399)
     // (outside active cells f1..f4 and omega are 0 and e2ok, e4ok false, so every
     // E is 0 as the reference's guard makes it)
-    const R dk = R(A.ds[k]), pk = R(A.pf[k]);
+    const R dk = A.ds[k], pk = A.pf[k];
     const R e1 = pk * X.f1;
     const R e2 = (X.e2ok && m > thr_gt<R>(1e-8)) ? m * X.f2 : R(0);
     const bool ent = X.omega > R(0) && X.h > A.tds_gt[k];
@@ -291,13 +297,13 @@ __device__ __forceinline__ void cell_class(Cell<R> &X, const StepArgs<R> &A, int
     // cg d is ~1e-6 of ak^2 and the difference keeps only the last digits of the
     // sqrt: FP64 evaluates it as written, FP32 as cg d / (sqrt(ak^2 + cg d) + ak),
     // equal in real arithmetic and free of the cancellation
-    const R ak = R(A.ak[k]);
+    const R ak = A.ak[k];
     R vf;
     if constexpr (sizeof(R) == sizeof(double)) {
-        vf = msqrt_pos(R(A.ak2[k]) + X.cg * dk) - ak;
+        vf = msqrt_pos(A.ak2[k] + X.cg * dk) - ak;
     } else {
         const R b = X.cg * dk;
-        vf = b * mrcp(msqrt_pos(R(A.ak2[k]) + b) + ak);
+        vf = b * mrcp(msqrt_pos(A.ak2[k] + b) + ak);
     }
     vf = vf * X.hind;
     if (((A.coarse1 >> k) & 1u) && X.shallow_v) vf = R(2e-3) + X.vshal * (vf - R(2e-3));
@@ -307,9 +313,13 @@ __device__ __forceinline__ void cell_class(Cell<R> &X, const StepArgs<R> &A, int
     // the running depth: in FP32 the depth before the step plus the increments so far
     const R hcur = sizeof(R) == sizeof(double) ? X.hn : (X.film ? hfilm : X.h) + X.dh_rest;
     if (hcur < thr_lt<R>(10 * kHfilm)) {
-        if constexpr (sizeof(R) == sizeof(double)) m += dt * e1 + u;
-        else m = add_unbiased(m, dt * e1 + u, cell_next_bits(X));
-        X.bud[B_DET] += (double)(dt * e1);
+        if constexpr (sizeof(R) == sizeof(double)) {
+            m += dt * e1 + u;
+            X.bud[B_DET] += (double)(dt * e1);
+        } else {
+            m = add_unbiased(m, dt * e1 + u, cell_next_bits(X));
+            X.bdet += dt * e1;
+        }
         u = R(0);
     } else {
         X.chtemp = u;
@@ -334,10 +344,11 @@ __device__ __forceinline__ void cell_class(Cell<R> &X, const StepArgs<R> &A, int
             const R inc = u * (em1 * (R(2) + em1)) + src * (R(1) + em1);
             u = add_unbiased(u, inc, cell_next_bits(X));
             const R dch = u - X.chtemp;
-            X.zn_c = X.zn_c - (double)(kbed * dch);
+            X.dz += kbed * dch;
             m = add_unbiased(m, dt * (e1 + e3) - dch, cell_next_bits(X));
         }
-        X.bud[B_DET] += (double)(dt * (e1 + e3));
+        if constexpr (sizeof(R) == sizeof(double)) X.bud[B_DET] += (double)(dt * (e1 + e3));
+        else X.bdet += dt * (e1 + e3);
         if (X.fail) {
             if constexpr (sizeof(R) == sizeof(double)) {
                 X.zn_c = X.zn_c - (double)(R(0.99) * m * kbed);
@@ -348,7 +359,7 @@ __device__ __forceinline__ void cell_class(Cell<R> &X, const StepArgs<R> &A, int
                 // products would not conserve the class's mass; the mass kept is
                 // rounded once and the rest moves into suspension
                 const R keep = R(0.01) * m, moved = m - keep;
-                X.zn_c = X.zn_c - (double)(moved * kbed);
+                X.dz += moved * kbed;
                 u = add_unbiased(u, moved, cell_next_bits(X));
                 m = keep;
             }
@@ -359,7 +370,8 @@ __device__ __forceinline__ void cell_class(Cell<R> &X, const StepArgs<R> &A, int
     const R dh = (u - X.chtemp) * kbed;
     if constexpr (sizeof(R) == sizeof(double)) X.hn = X.hn + dh;
     else X.dh_rest += dh;
-    X.bud[B_EXCH] += (double)dh;
+    if constexpr (sizeof(R) == sizeof(double)) X.bud[B_EXCH] += (double)dh;
+    else X.bexch += dh;
     X.totm += m;
     X.depn += m * R(1.0 / kRhos);
     X.left_suspended = X.left_suspended || u != R(0);
@@ -384,7 +396,12 @@ __device__ __forceinline__ void cell_end(Cell<R> &X, const StepArgs<R> &A, doubl
     const int nx = A.nx, ny = A.ny, gi = X.gi, gj = X.gj, S = A.S;
     const R dt = X.dt;
     R hn = X.hn, hun = X.hun, hvn = X.hvn;
-    if constexpr (sizeof(R) == sizeof(float)) hn = add_unbiased(X.film ? hfilm : X.h, X.dh_rest, cell_next_bits(X));
+    if constexpr (sizeof(R) == sizeof(float)) {
+        hn = add_unbiased(X.film ? hfilm : X.h, X.dh_rest, cell_next_bits(X));
+        X.zn_c = X.zn_c - (double)X.dz;
+        X.bud[B_DET] += (double)X.bdet;
+        X.bud[B_EXCH] += (double)X.bexch;
+    }
 
     // dry-cell clamp (SWEHR.c:1532-1551)
     if (!(hn > hfilm) && X.own) {

@@ -95,9 +95,11 @@ struct StepArgs {
     int tiles_i, tiles_j, nblocks, blocks_per_xcd;
     int bud_slots;  // budget slots allocated: one per wave of every tile, any tile shape
     int stage_cap;  // rows of the stage / saveflow series
-    double ds[kMaxClasses], pf[kMaxClasses];
-    // per-class settling constants a_k = 13.95 nu / d_k and a_k^2 (SWEHR.c:435)
-    double ak[kMaxClasses], ak2[kMaxClasses];
+    // per-class diameter and fraction, and the settling constants a_k = 13.95 nu / d_k
+    // and a_k^2 (SWEHR.c:435), formed in FP64 and stored in the step's precision (the
+    // class loop reads them so, with no conversion per class)
+    R ds[kMaxClasses], pf[kMaxClasses];
+    R ak[kMaxClasses], ak2[kMaxClasses];
     // bit k: d_k > 0.0001, resp. d_k > 0.002, compared in FP64 as the reference
     // compares them (the shallow-flow settling rules, SWEHR.c:1385-1388)
     unsigned coarse1, coarse2;

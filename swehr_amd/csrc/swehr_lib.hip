@@ -430,13 +430,15 @@ void fill_args(swehr_handle *h, StepArgs<R> &A, const double *ds, const double *
     A.bud_slots = h->bud_slots;
     A.coarse1 = A.coarse2 = 0;
     for (int k = 0; k < kMaxClasses; k++) {
-        A.ds[k] = k < h->S ? ds[k] : 1.0;
-        A.pf[k] = k < h->S ? pf[k] : 0.0;
-        A.ak[k] = 13.95 * kNu / A.ds[k];
-        A.ak2[k] = A.ak[k] * A.ak[k];
-        if (A.ds[k] > 0.0001) A.coarse1 |= 1u << k;
-        if (A.ds[k] > 0.002) A.coarse2 |= 1u << k;
-        A.tds_gt[k] = thr_gt<R>(A.ds[k]);
+        const double d = k < h->S ? ds[k] : 1.0;
+        const double ak = 13.95 * kNu / d;
+        A.ds[k] = (R)d;
+        A.pf[k] = (R)(k < h->S ? pf[k] : 0.0);
+        A.ak[k] = (R)ak;
+        A.ak2[k] = (R)(ak * ak);
+        if (d > 0.0001) A.coarse1 |= 1u << k;
+        if (d > 0.002) A.coarse2 |= 1u << k;
+        A.tds_gt[k] = thr_gt<R>(d);
     }
     A.t9h0_lt = thr_lt<R>(9 * p.h0);
     A.t9h0_gt = thr_gt<R>(9 * p.h0);
