@@ -125,6 +125,8 @@ struct Engine {
     DevCtrl *h_gather = nullptr;  // pinned
     size_t cap_members = 0, cap_map = 0;
     int n_map = 0, per_xcd = 0;
+    // the wave-speed map (instance, strip tile) follows the step map in d_map
+    int n_map1 = 0, per_xcd1 = 0;
     size_t lds = 0;
     int max_classes = 0;  // largest S of the members
     hipGraph_t graph = nullptr;
@@ -208,13 +210,16 @@ struct Engine {
     {
         drop_graph();
         const size_t n = members.size();
-        size_t nmap = 0;
-        for (swehr_handle *h : members) nmap += (size_t)h->nblocks;
-        int rc = reserve(std::max<size_t>(n, 1), std::max<size_t>(nmap, 1));
+        size_t nmap = 0, nmap1 = 0;
+        for (swehr_handle *h : members) {
+            nmap += (size_t)h->nblocks;
+            nmap1 += (size_t)k1_tiles(h->nx, h->ny);
+        }
+        int rc = reserve(std::max<size_t>(n, 1), std::max<size_t>(nmap + nmap1, 1));
         if (rc) return rc;
         std::vector<unsigned char> args(n * arg_size());
         std::vector<TileRef> map;
-        map.reserve(nmap);
+        map.reserve(nmap + nmap1);
         std::vector<int> insts(n);
         std::vector<DevCtrl *> ctrls(n);
         lds = 0;
@@ -229,8 +234,12 @@ struct Engine {
             lds = std::max(lds, tile_lds_bytes(tile, precision, h->S));
             max_classes = std::max(max_classes, h->S);
         }
+        for (size_t i = 0; i < n; i++)
+            for (int t = 0; t < k1_tiles(members[i]->nx, members[i]->ny); t++) map.push_back(TileRef{(int)i, t});
         n_map = (int)nmap;
         per_xcd = (n_map + 7) / 8;
+        n_map1 = (int)nmap1;
+        per_xcd1 = (n_map1 + 7) / 8;
         if (n) {
             HIPCHK(hipMemcpyAsync(d_args, args.data(), args.size(), hipMemcpyHostToDevice, stream));
             HIPCHK(hipMemcpyAsync(d_map, map.data(), map.size() * sizeof(TileRef), hipMemcpyHostToDevice, stream));
@@ -254,7 +263,7 @@ struct Engine {
         int rc = SWEHR_OK;
         for (int s = 0; s < steps && rc == SWEHR_OK; s++) {
             (void)hipEventRecord(ev[0], stream);
-            launch_batch_wave(tile, precision, blocks, stream, d_args, d_map, n_map, per_xcd);
+            launch_batch_wave_strip(precision, 8 * per_xcd1, stream, d_args, d_map + n_map, n_map1, per_xcd1);
             (void)hipEventRecord(ev[1], stream);
             launch_batch_step(tile, precision, blocks, lds, stream, d_args, d_map, n_map, per_xcd, max_classes);
             (void)hipEventRecord(ev[2], stream);
@@ -287,7 +296,7 @@ struct Engine {
         tile_shape(tile, &ti, &tj);
         const int blocks = 8 * per_xcd;
         for (int s = 0; s < steps; s++) {
-            launch_batch_wave(tile, precision, blocks, stream, d_args, d_map, n_map, per_xcd);
+            launch_batch_wave_strip(precision, 8 * per_xcd1, stream, d_args, d_map + n_map, n_map1, per_xcd1);
             launch_batch_step(tile, precision, blocks, lds, stream, d_args, d_map, n_map, per_xcd, max_classes);
             if (precision == SWEHR_FP64)
                 hipLaunchKernelGGL((batch_control_kernel<double>), dim3(n), dim3(256), 0, stream,

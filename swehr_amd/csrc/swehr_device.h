@@ -604,25 +604,20 @@ constexpr int kK1Rows = 4;               // rows per wave
 constexpr int kK1TI = 4 * kK1Rows, kK1TJ = 62;  // rows and computed columns per block
 __host__ __device__ inline int k1_tiles(int nx, int ny) { return ((nx + kK1TI - 1) / kK1TI) * ((ny + kK1TJ - 1) / kK1TJ); }
 
+// The wave speeds of strip tile `tile` of an instance (the strip geometry above; the
+// block's 256 threads are 4 waves of kK1Rows rows each): the largest face speed of this
+// thread's cells.  `view` gives the step count and the inflow state.
 template <typename R>
-__global__ __launch_bounds__(256) void wave_speed_kernel(StepArgs<R> A)
+__device__ __forceinline__ R wave_speed_strip(const StepArgs<R> &A, const K1View &view, int tile)
 {
     constexpr int RPW = kK1Rows;
-    glob<DevCtrl> *C = gmem(A.ctrl);
-    const int b = blockIdx.x;
-    const K1View view = k1_view(C);
-    // block 0 also commits the previous step (its stage records read that step's stored
-    // cells, which the launch boundary has made visible)
-    if (!view.active && b != 0) return;
     const long long steps = view.steps;
     const int p = (int)(steps & 1), p3 = (int)(steps % 3);
     const InflowState in = view.in;
     const int nx = A.nx, ny = A.ny;
-    const int tiles_j = (ny + kK1TJ - 1) / kK1TJ, ntiles = k1_tiles(nx, ny);
-    const int per = (int)(gridDim.x >> 3);
-    const int tile = (b & 7) * per + (b >> 3);
+    const int tiles_j = (ny + kK1TJ - 1) / kK1TJ;
     R s = R(0);
-    if (view.active && tile < ntiles) {
+    {
         const int ti = tile / tiles_j, tj = tile - ti * tiles_j;
         const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
         const int j = tj * kK1TJ - 1 + lane;  // this lane's column (-1 or >= ny at the edges)
@@ -703,13 +698,30 @@ __global__ __launch_bounds__(256) void wave_speed_kernel(StepArgs<R> A)
             }
         }
     }
+    return s;
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void wave_speed_kernel(StepArgs<R> A)
+{
+    glob<DevCtrl> *C = gmem(A.ctrl);
+    const int b = blockIdx.x;
+    const K1View view = k1_view(C);
+    // block 0 also commits the previous step (its stage records read that step's stored
+    // cells, which the launch boundary has made visible)
+    if (!view.active && b != 0) return;
+    const int ntiles = k1_tiles(A.nx, A.ny);
+    const int per = (int)(gridDim.x >> 3);
+    const int tile = (b & 7) * per + (b >> 3);
+    R s = R(0);
+    if (view.active && tile < ntiles) s = wave_speed_strip(A, view, tile);
     if (view.active) {
         // the step kernel's budget partials of the previous step
         const long long nterms = (long long)A.bud_slots * NB_TERMS;
         for (long long g = (long long)b * blockDim.x + threadIdx.x; g < nterms;
              g += (long long)gridDim.x * blockDim.x)
             fold_budget(gmem(A.bud), nterms, g);
-        block_max_to((unsigned long long *)C->smax_slots[steps & 1], b, (double)s);
+        block_max_to((unsigned long long *)C->smax_slots[view.steps & 1], b, (double)s);
     }
     if (b == 0) {
         __shared__ int s_commit;
@@ -719,7 +731,7 @@ __global__ __launch_bounds__(256) void wave_speed_kernel(StepArgs<R> A)
     }
 }
 
-// Wave speeds of one TI x TJ tile (ensemble path: one block per tile).
+// Wave speeds of one TI x TJ tile (the persistent ensemble mode, one tile at a time).
 template <typename R, int TI, int TJ>
 __device__ __forceinline__ void wave_speed_tile(const StepArgs<R> &A, int tile)
 {
@@ -1332,15 +1344,27 @@ struct TileRef {
     int inst, tile;
 };
 
-template <typename R, int TI, int TJ>
-__global__ __launch_bounds__(TI *TJ) void batch_wave_kernel(const StepArgs<R> *args, const TileRef *map, int n,
-                                                            int per_xcd)
+// K1 of the ensemble on strip tiles: one block per (instance, strip tile) entry of the
+// wave-speed map.  Each block also folds its share of its instance's budget partials
+// (the strip tiles of an instance split the slots between them).
+template <typename R>
+__global__ __launch_bounds__(256) void batch_wave_strip_kernel(const StepArgs<R> *args, const TileRef *map, int n,
+                                                                int per_xcd)
 {
     const int b = blockIdx.x;
     const int k = (b & 7) * per_xcd + (b >> 3);
     if (k >= n) return;
     const TileRef r = map[k];
-    wave_speed_tile<R, TI, TJ>(args[r.inst], r.tile);
+    const StepArgs<R> &A = args[r.inst];
+    glob<DevCtrl> *C = gmem(A.ctrl);
+    const K1View view = k1_view(C);
+    if (!view.active) return;
+    const R s = wave_speed_strip(A, view, r.tile);
+    const long long nterms = (long long)A.bud_slots * NB_TERMS;
+    const long long share = (nterms + k1_tiles(A.nx, A.ny) - 1) / k1_tiles(A.nx, A.ny);
+    const long long g0 = (long long)r.tile * share, g1 = g0 + share < nterms ? g0 + share : nterms;
+    for (long long g = g0 + threadIdx.x; g < g1; g += blockDim.x) fold_budget(gmem(A.bud), nterms, g);
+    block_max_to((unsigned long long *)C->smax_slots[view.steps & 1], r.tile, (double)s);
 }
 
 template <typename R, int TI, int TJ, bool SPLIT>

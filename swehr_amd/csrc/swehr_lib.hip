@@ -115,17 +115,6 @@ size_t tile_lds_bytes(int id, int prec, int S)
     return prec == SWEHR_FP64 ? tile_lds<double>(id, S) : tile_lds<float>(id, S);
 }
 
-template <typename R>
-static void batch_wave_t(int id, int blocks, hipStream_t s, const StepArgs<R> *a, const TileRef *map, int n, int per)
-{
-    switch (id) {
-    case 1: hipLaunchKernelGGL((batch_wave_kernel<R, 8, 32>), dim3(blocks), dim3(256), 0, s, a, map, n, per); break;
-    case 2: hipLaunchKernelGGL((batch_wave_kernel<R, 4, 64>), dim3(blocks), dim3(256), 0, s, a, map, n, per); break;
-    case 3: hipLaunchKernelGGL((batch_wave_kernel<R, 16, 16>), dim3(blocks), dim3(256), 0, s, a, map, n, per); break;
-    default: hipLaunchKernelGGL((batch_wave_kernel<R, 4, 32>), dim3(blocks), dim3(128), 0, s, a, map, n, per); break;
-    }
-}
-
 template <typename R, bool SP>
 static void batch_step_t(int id, int blocks, size_t lds, hipStream_t s, const StepArgs<R> *a, const TileRef *map,
                          int n, int per)
@@ -138,11 +127,15 @@ static void batch_step_t(int id, int blocks, size_t lds, hipStream_t s, const St
     }
 }
 
-void launch_batch_wave(int id, int prec, int blocks, hipStream_t s, const void *args, const TileRef *map, int n,
-                       int per_xcd)
+void launch_batch_wave_strip(int prec, int blocks, hipStream_t s, const void *args, const TileRef *map, int n,
+                             int per_xcd)
 {
-    if (prec == SWEHR_FP64) batch_wave_t(id, blocks, s, (const StepArgs<double> *)args, map, n, per_xcd);
-    else batch_wave_t(id, blocks, s, (const StepArgs<float> *)args, map, n, per_xcd);
+    if (prec == SWEHR_FP64)
+        hipLaunchKernelGGL((batch_wave_strip_kernel<double>), dim3(blocks), dim3(256), 0, s,
+                           (const StepArgs<double> *)args, map, n, per_xcd);
+    else
+        hipLaunchKernelGGL((batch_wave_strip_kernel<float>), dim3(blocks), dim3(256), 0, s,
+                           (const StepArgs<float> *)args, map, n, per_xcd);
 }
 
 bool batch_uses_split(int prec, int id, int max_classes, size_t lds)

@@ -63,9 +63,11 @@ void tile_shape(int id, int *ti, int *tj);
 size_t tile_lds_bytes(int id, int prec, int S);
 // Largest LDS a step-kernel workgroup may use.
 constexpr size_t kLdsLimit = 64 * 1024;
-// Ensemble launches of K1 / K2 for tile shape id (one block per map entry).
-void launch_batch_wave(int id, int prec, int blocks, hipStream_t s, const void *args, const TileRef *map, int n,
-                       int per_xcd);
+// Ensemble launch of K2 for tile shape id (one block per map entry).
+// Ensemble K1 on strip tiles: one block per entry of the wave-speed map (instance,
+// strip tile), blocks a multiple of 8.
+void launch_batch_wave_strip(int prec, int blocks, hipStream_t s, const void *args, const TileRef *map, int n,
+                             int per_xcd);
 // max_classes: the largest S of the members (selects the FP64 two-chunk variant above
 // kLdsClasses<double>)
 void launch_batch_step(int id, int prec, int blocks, size_t lds, hipStream_t s, const void *args, const TileRef *map,
