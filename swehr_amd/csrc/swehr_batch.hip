@@ -82,7 +82,7 @@ __global__ __launch_bounds__(TI *TJ, (kStepWaves<R, SPLIT>)) void batch_persiste
             const int k = xcd * per_xcd + q;
             if (k < n) {
                 const TileRef r = map[k];
-                step_tile<R, TI, TJ, SPLIT, fp64_own<TI, TJ, true>()>(args[r.inst], r.tile, smem_raw);
+                step_tile<R, TI, TJ, SPLIT, fp64_own<TI, TJ, true>(), true>(args[r.inst], r.tile, smem_raw);
             }
             __syncthreads();
         }

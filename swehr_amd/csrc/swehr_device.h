@@ -742,15 +742,18 @@ __device__ __forceinline__ void wave_speed_tile(const StepArgs<R> &A, int tile)
     const int p = (int)(steps & 1), p3 = (int)(steps % 3);
     const InflowState in = view.in;
     const int ti = tile / A.tiles_j, tj = tile - (tile / A.tiles_j) * A.tiles_j;
-    const int il = threadIdx.x / TJ, jl = threadIdx.x - (threadIdx.x / TJ) * TJ;
+    // (the thread index through an empty asm statement, as in step_tile's LOOP form:
+    // this function runs in the persistent kernel's loop over tiles)
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int il = tid / TJ, jl = tid - (tid / TJ) * TJ;
     const int i = ti * TI + il, j = tj * TJ + jl;
     R s = R(0);
     if (i < A.nx && j < A.ny)
         s = wave_speed_cell(A, gmem(A.h[p]), gmem(A.hu[p]), gmem(A.hv[p]), gmem(A.z[(p3 + 2) % 3]), i, j, in);
     // the step kernel's budget partials of this tile's slots (previous step)
-    if ((int)threadIdx.x < (TI * TJ / 64) * NB_TERMS)
-        fold_budget(gmem(A.bud), (long long)A.bud_slots * NB_TERMS,
-                    (long long)tile * (TI * TJ / 64) * NB_TERMS + threadIdx.x);
+    if (tid < (TI * TJ / 64) * NB_TERMS)
+        fold_budget(gmem(A.bud), (long long)A.bud_slots * NB_TERMS, (long long)tile * (TI * TJ / 64) * NB_TERMS + tid);
     block_max_to((unsigned long long *)C->smax_slots[steps & 1], tile, (double)s);
 }
 
@@ -828,7 +831,11 @@ struct TileGeom {
 // per CU instead of 5, it is compiled for 4 waves per SIMD and spends the registers
 // this frees on the own cell's ch_k and M_k of the first classes, which are then read
 // once (with the halo staging) instead of again in the class loop.
-template <typename R, int TI, int TJ, bool SPLIT, int KOWN>
+// LOOP: the body of a loop over tiles (the persistent ensemble kernel).  The thread
+// index is then passed through an empty asm statement, so that the compiler cannot
+// hoist the per-thread index arithmetic out of the loop, where it would stay live
+// through every tile's step and crowd its registers.
+template <typename R, int TI, int TJ, bool SPLIT, int KOWN, bool LOOP = false>
 __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsigned char *smem_raw)
 {
     using T = TileGeom<R, TI, TJ>;
@@ -871,7 +878,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
 
     const R hfilm = R(kHfilm);
     const R inv_rhos = R(1.0 / kRhos);
-    const int tid = threadIdx.x;
+    int tid = threadIdx.x;
+    if constexpr (LOOP) asm volatile("" : "+v"(tid));
     if (sizeof(R) == sizeof(double) && tid < 32) s_exp2[tid] = kExp2Frac[tid];
     if (sizeof(R) == sizeof(double) && tid < 69) s_log[tid] = (&kLogTab[0][0])[tid];
 
