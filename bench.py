@@ -515,13 +515,16 @@ def bench_batched(hip, count: int, scale_max: float, tend: float, plain_every: i
 def serve(srv, reqs, arrivals):
     """Submit reqs at the given arrival times (seconds from now; all 0 for a closed batch)
     and pump the server until every request has finished.  Returns (latencies in
-    seconds, completion minus arrival, per request; work in cell-class-steps; wall)."""
+    seconds, completion minus arrival, per request; work in cell-class-steps; wall;
+    (seconds, cell-class-steps) when the queue first stood empty after the last
+    submission, from the server's work counter, which counts unfinished requests too)."""
     import numpy as np
 
     n = len(reqs)
     lat, work = {}, 0.0
     t0 = time.perf_counter()
-    sub, nxt = {}, 0
+    w0 = srv.work()
+    sub, nxt, drained = {}, 0, None
     while len(lat) < n:
         now = time.perf_counter() - t0
         while nxt < n and arrivals[nxt] <= now:
@@ -529,6 +532,8 @@ def serve(srv, reqs, arrivals):
             sub[rid] = (nxt, t0 + arrivals[nxt])
             nxt += 1
         running, queued, finished = srv.pump(16)
+        if drained is None and nxt == n and queued == 0:
+            drained = (time.perf_counter() - t0, srv.work() - w0)
         while True:
             d = srv.next_done()
             if d is None:
@@ -541,18 +546,22 @@ def serve(srv, reqs, arrivals):
             srv.release(rid)
         if running == 0 and queued == 0 and nxt < n:
             time.sleep(max(0.0, min(0.01, t0 + arrivals[nxt] - time.perf_counter())))
-    return np.array([lat[k] for k in range(n)]), work, time.perf_counter() - t0
+    return np.array([lat[k] for k in range(n)]), work, time.perf_counter() - t0, drained
 
 
 def bench_serving(hip, nreq: int, load: float, cap_requests: int, mem_gib: float, slots: int, seed: int = 5):
     """Solve requests of the configured mix (square grids 128^2-2048^2, 3-10 classes,
     durations U[600, 3600] s of simulated time) through the continuous-batching server
     (FP32 state, FP64 budgets).  Capacity first: cap_requests requests of the same mix
-    submitted at once, work completed over the makespan (a lower bound of the saturated
-    rate: the makespan includes the drain, when fewer requests are left than slots).
+    (more than the server's slots) submitted at once; the server runs full until its
+    queue empties, and the capacity is the work it stepped over that window (its work
+    counter, unfinished requests included) per second.  The makespan rate of the whole
+    batch is reported beside it: its drain, when fewer requests are left than slots,
+    lasts as long as the slowest request, whose step count is chaotic (a steep small
+    grid can take 10^5-10^6 steps, and FP32 rounding alone changes the count 2-3 fold).
     Then an open loop: nreq requests arriving as a Poisson stream at load x capacity,
-    in requests per second of the capacity run's mean work per request; latency =
-    completion - arrival (wall clock)."""
+    in requests per second of the batch's mean work per request; latency = completion -
+    arrival (wall clock)."""
     import numpy as np
 
     from swehr_amd import _abi
@@ -562,20 +571,23 @@ def bench_serving(hip, nreq: int, load: float, cap_requests: int, mem_gib: float
     rng = np.random.default_rng(seed)
     cap_reqs = [config_c5_request(rng, 10000 + k) for k in range(cap_requests)]
     reqs = [config_c5_request(rng, k) for k in range(nreq)]
+    if cap_requests <= slots:
+        raise ValueError("the capacity batch must exceed the server's slots (%d)" % slots)
     with sv.Server(mem_bytes=int(mem_gib * 2 ** 30), max_slots=slots, precision=_abi.FP32) as srv:
-        _, cap_work, cap_wall = serve(srv, cap_reqs, np.zeros(cap_requests))
-        capacity = cap_work / cap_wall
+        _, cap_work, cap_wall, (sat_s, sat_work) = serve(srv, cap_reqs, np.zeros(cap_requests))
+        capacity = sat_work / sat_s
         per_req = cap_work / cap_requests
         rate = load * capacity / per_req
-        log("C5 capacity: %d requests in %.2f s, %.3e cell-class-steps/s; open loop at %.3f requests/s"
-            % (cap_requests, cap_wall, capacity, rate))
+        log("C5 capacity: %.3e cell-class-steps/s over the first %.2f s (server full); the batch of %d "
+            "requests took %.2f s; open loop at %.3f requests/s" % (capacity, sat_s, cap_requests, cap_wall, rate))
         arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
-        L, work, wall = serve(srv, reqs, arrivals)
+        L, work, wall, _ = serve(srv, reqs, arrivals)
     sizes = sorted(set(c.nx for c in reqs))
     return {"workload": "%d requests, Poisson at %.3f/s = %.2f x measured capacity, sizes %s (weights "
                         ".4/.3/.15/.1/.05), S 3-10, durations U[600,3600] s (unscaled)"
                         % (nreq, rate, load, "/".join("%d^2" % n for n in sizes)),
-            "capacity": {"requests": cap_requests, "wall_s": cap_wall, "cell_class_steps_per_s": capacity,
+            "capacity": {"requests": cap_requests, "saturated_s": sat_s, "cell_class_steps_per_s": capacity,
+                         "batch_wall_s": cap_wall, "batch_cell_class_steps_per_s": cap_work / cap_wall,
                          "mean_work_per_request": per_req, "requests_per_s": capacity / per_req},
             "arrival_rate_per_s": rate, "load": load,
             "throughput_cell_class_steps_per_s": work / wall, "wall_s": wall,
@@ -622,8 +634,9 @@ def main():
     ap.add_argument("--c4-scale-max", type=float, default=16.0)
     ap.add_argument("--c5-requests", type=int, default=40)
     ap.add_argument("--c5-load", type=float, default=0.7, help="C5 arrival rate as a fraction of measured capacity")
-    ap.add_argument("--c5-capacity-requests", type=int, default=24,
-                    help="requests submitted at once to measure the server's capacity")
+    ap.add_argument("--c5-capacity-requests", type=int, default=48,
+                    help="requests submitted at once to measure the server's capacity (more than --c5-slots: "
+                         "the server runs full until its queue empties)")
     ap.add_argument("--c5-mem-gib", type=float, default=64.0)
     ap.add_argument("--c5-slots", type=int, default=32)
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",

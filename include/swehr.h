@@ -310,6 +310,12 @@ int swehr_server_request(swehr_server *s, int64_t id, swehr_handle **h);
 int swehr_server_release(swehr_server *s, int64_t id);
 /* Device memory held by admitted requests and the budget (bytes). */
 int swehr_server_stats(swehr_server *s, int64_t *mem_used, int64_t *mem_budget);
+/* Work stepped so far over every request the server has run, finished or running:
+ * the sum of cells x size classes x steps (cell-class-steps), updated after each graph
+ * of a pump.  Sampled between pumps it gives the server's throughput over any window,
+ * unfinished requests included.  No counterpart in the reference (it has no serving
+ * loop). */
+int swehr_server_work(swehr_server *s, int64_t *cell_class_steps);
 /* Evict a running request: its whole device state is checkpointed into pinned host
  * memory (as swehr_snapshot), its device memory freed, and it joins the back of the
  * queue; when admitted again it is restored and continues exactly where it stopped.

@@ -58,8 +58,10 @@ __global__ void set_limits_kernel(DevCtrl *const *ctrls, int n, long long add)
 // whose shared memory the next tile reuses.  The grid holds at most as many blocks as
 // are resident at once (cooperative launch), so every grid barrier is reached.
 // Register cost: the step body inside a loop over tiles does not fit the 3-wave (FP64)
-// or 5-wave (FP32) budget that one tile per block fits without a spill; the compiler
-// spills ~420 bytes per lane here (and more with the stages as separate calls).
+// or 5-wave (FP32) budget that one tile per block fits without a spill.  Built as it
+// is (this file without machine LICM, the thread index laundered in the loop body:
+// step_tile's LOOP form) the compiler spills 44-72 bytes per lane, against ~420 with
+// the step's constants and index arithmetic hoisted out of the loop.
 template <typename R, int TI, int TJ, bool SPLIT>
 __global__ __launch_bounds__(TI *TJ, (kStepWaves<R, SPLIT>)) void batch_persistent_kernel(
     const StepArgs<R> *__restrict__ args, const TileRef *__restrict__ map, int n, int per_xcd,
@@ -684,6 +686,11 @@ struct swehr_server {
     std::map<int64_t, Request> live;       // inputs of the running requests (for re-admission)
     std::map<int64_t, int64_t> slice;      // steps taken since (re)admission
     int64_t quantum = 0, evictions = 0;
+    // cell-class-steps stepped so far over every request, and each running request's
+    // step count when it was last added in (kept across an eviction: the restored
+    // instance resumes at the count it was checkpointed with)
+    int64_t work = 0;
+    std::map<int64_t, long long> seen;
 };
 
 namespace {
@@ -874,6 +881,9 @@ extern "C" int swehr_server_pump(swehr_server *s, int steps, int *running, int *
             for (size_t i = 0; i < E.members.size(); i++) {
                 swehr_handle *h = E.members[i];
                 const DevCtrl &c = E.h_gather[i];
+                long long &seen = s->seen[s->ids[h]];
+                s->work += (int64_t)h->nx * h->ny * h->S * (c.steps - seen);
+                seen = c.steps;
                 if (c.done) {
                     const int64_t id = s->ids[h];
                     swehr_step_info info;
@@ -890,6 +900,7 @@ extern "C" int swehr_server_pump(swehr_server *s, int steps, int *running, int *
                     s->running.erase(id);
                     s->live.erase(id);
                     s->slice.erase(id);
+                    s->seen.erase(id);
                     s->finished[id] = h;
                     s->done_fifo.push_back({id, info});
                 } else {
@@ -906,6 +917,13 @@ extern "C" int swehr_server_pump(swehr_server *s, int steps, int *running, int *
     if (running) *running = (int)s->running.size();
     if (queued) *queued = (int)s->queue.size();
     if (finished) *finished = (int)s->finished.size();
+    return SWEHR_OK;
+}
+
+extern "C" int swehr_server_work(swehr_server *s, int64_t *cell_class_steps)
+{
+    if (!s || !cell_class_steps) return SWEHR_ERR_BAD_INPUT;
+    *cell_class_steps = s->work;
     return SWEHR_OK;
 }
 

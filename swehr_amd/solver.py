@@ -59,6 +59,7 @@ def lib():
     L.swehr_instance_bytes_for.restype = C.c_int64
     L.swehr_instance_bytes_for.argtypes = [C.POINTER(_abi.Params), C.c_int, C.c_int]
     L.swehr_server_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.swehr_server_work.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
     L.swehr_server_evict.argtypes = [C.c_void_p, C.c_int64]
     L.swehr_server_set_quantum.argtypes = [C.c_void_p, C.c_int64]
     L.swehr_server_evictions.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
@@ -453,6 +454,12 @@ class Server:
         used, budget = C.c_int64(0), C.c_int64(0)
         check("swehr_server_stats", lib().swehr_server_stats(self._s, C.byref(used), C.byref(budget)))
         return used.value, budget.value
+
+    def work(self) -> int:
+        """Cell-class-steps stepped so far over every request, finished or running."""
+        w = C.c_int64(0)
+        check("swehr_server_work", lib().swehr_server_work(self._s, C.byref(w)))
+        return w.value
 
     def release(self, rid: int):
         check("swehr_server_release", lib().swehr_server_release(self._s, int(rid)))

@@ -133,7 +133,9 @@ def test_ensemble_step_limit_and_resume():
 def test_server_continuous_batching_results_equal_alone():
     """Requests admitted as slots and memory free up; each result equals the same case
     run alone.  Finished requests hold their device memory until released, so the
-    client checks and releases each one as it completes."""
+    client checks and releases each one as it completes.  The server's work counter
+    never falls, never lags the finished requests' cells x classes x steps, and equals
+    their sum once all have finished."""
     rng = np.random.default_rng(31)
     reqs = [config_c5_request(rng, k, sizes=(32, 48, 64), weights=(1, 1, 1), tmin=30.0, tmax=90.0) for k in range(6)]
     sizes = [sv.instance_bytes_for(c, _abi.FP32) for c in reqs]
@@ -142,6 +144,8 @@ def test_server_continuous_batching_results_equal_alone():
     with sv.Server(mem_bytes=budget, max_slots=4, precision=_abi.FP32) as srv:
         ids = {srv.submit(c): c for c in reqs}
         max_running = 0
+        done_work, last_work = 0, srv.work()
+        assert last_work == 0
         for _ in range(10000):
             running, queued, finished = srv.pump(16)
             used, _ = srv.stats()
@@ -153,6 +157,7 @@ def test_server_continuous_batching_results_equal_alone():
                     break
                 rid, info = d
                 c = ids[rid]
+                done_work += c.nx * c.ny * c.nclass * info["steps"]
                 r = srv.result(rid)
                 with sv.Solver(c, precision=_abi.FP32) as s:
                     s.run()
@@ -160,9 +165,13 @@ def test_server_continuous_batching_results_equal_alone():
                     _same(r, s, c)
                 srv.release(rid)
                 checked += 1
+            w = srv.work()
+            assert last_work <= w and done_work <= w
+            last_work = w
             if checked == len(reqs):
                 break
         assert checked == len(reqs)
+        assert srv.work() == done_work
         assert 2 <= max_running <= min(4, budget // min(sizes))  # slots and memory budget
         assert max_running < len(reqs)
 
@@ -220,9 +229,11 @@ def test_server_sized_by_free_device_memory():
 
 
 def _drain(srv, ids, reqs_by_id, max_pumps=20000):
-    """Pump until every request finished; check each against its alone run and release it.
-    Returns the largest number of requests running at once."""
-    done, max_running = 0, 0
+    """Pump until every request finished; check each against its alone run and release it,
+    and the server's work counter against the requests' cells x classes x steps (each
+    step counted once, across evictions and re-admissions).  Returns the largest number
+    of requests running at once."""
+    done, max_running, work = 0, 0, 0
     for _ in range(max_pumps):
         running, queued, finished = srv.pump(16)
         max_running = max(max_running, running)
@@ -232,6 +243,7 @@ def _drain(srv, ids, reqs_by_id, max_pumps=20000):
                 break
             rid, info = d
             c = reqs_by_id[rid]
+            work += c.nx * c.ny * c.nclass * info["steps"]
             r = srv.result(rid)
             with sv.Solver(c, precision=srv.precision) as s:
                 s.run()
@@ -240,6 +252,7 @@ def _drain(srv, ids, reqs_by_id, max_pumps=20000):
             srv.release(rid)
             done += 1
         if done == len(ids):
+            assert srv.work() == work
             return max_running
     raise AssertionError("requests did not finish")
 
