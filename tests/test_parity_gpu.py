@@ -215,11 +215,10 @@ def test_fp64_many_classes_match_oracle(S, nx, ny):
         # the finest classes' deposited mass: its step increments are differences of
         # nearly equal numbers (measured 6e-12..1.2e-11, the FMA-compiled oracle is
         # 6.9e-11 from the oracle on the 10-class case); per field within 4x the floor
-        from parity_util import fp64_violations, rounding_floor
+        from parity_util import fp64_check
 
-        fl = rounding_floor(case, 400, o.state())
+        bad, fl = fp64_check(case, 400, o.state(), errs, RTOL_FP64_SHORT, 4.0)
         print("rounding floor on %s %.3e" % (k, fl[k]))
-        bad = fp64_violations(errs, fl, RTOL_FP64_SHORT, 4.0)
         assert not bad, bad
     st_g, _ = s.stage()
     st_o, _ = o.stage()
