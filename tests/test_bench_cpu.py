@@ -28,3 +28,24 @@ def test_bytes_per_cell_matches_the_algorithmic_count():
     assert bench.bytes_per_cell(10, 4) == 228
     assert bench.bytes_per_cell(5, 4) == 148
     assert bench.bytes_per_cell(7, 8, ks_raster=False) == 336
+
+
+def test_c5_requests_relief_scales_with_extent():
+    """Serving requests carry the relief of a window of the 4096 m fractal surface,
+    (5-20 m) x (extent / 4096 m)^0.8, on the 5 % tilt: steep small grids (the full 5-20 m
+    on 128 m) drove the explicit scheme to speeds of ~1000 m/s and 10^5-10^6 steps."""
+    import numpy as np
+
+    from swehr_amd.inputs import config_c5_request
+
+    rng = np.random.default_rng(5)
+    seen = set()
+    for k in range(40):
+        c = config_c5_request(rng, k, sizes=(128, 256, 512), weights=(1, 1, 1))
+        j = np.arange(c.ny, dtype=np.float64)
+        rough = c.topo - 0.05 * c.dx * j[None, :]  # the fractal part, tilt removed
+        relief = float(rough.max() - rough.min())
+        cap = 20.0 * (c.nx * c.dx / 4096.0) ** 0.8
+        assert 0.25 * cap <= relief <= cap * (1 + 1e-12), (c.nx, relief, cap)
+        seen.add(c.nx)
+    assert seen == {128, 256, 512}
