@@ -182,3 +182,14 @@ def test_parallel_parser_reads_like_fscanf():
     assert np.array_equal(np.signbit(out[:len(v)]), np.signbit(v))
     bad = b"1.0 2.0 3x 4.0"
     assert p(bad, len(bad), out.ctypes.data_as(C.POINTER(C.c_double)), 4) == -1
+
+
+def test_driver_reports_a_missing_input_file(tmp_path):
+    """The drop-in program stops with a message naming the file and a nonzero status
+    where an input file is missing (it reads every input before it touches a GPU)."""
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "swehr_amd", "bin", "swehr")
+    if not os.path.exists(exe):
+        pytest.skip("drop-in program not built")
+    r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0
+    assert "input.txt" in (r.stderr + r.stdout)
