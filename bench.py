@@ -557,9 +557,8 @@ def bench_serving(hip, nreq: int, load: float, cap_requests: int, mem_gib: float
     queue empties, and the capacity is the work it stepped over that window (its work
     counter, unfinished requests included) per second.  The makespan rate of the whole
     batch is reported beside it: its drain, when fewer requests are left than slots,
-    lasts as long as the slowest request, whose step count is chaotic (a steep small
-    grid can take 10^5-10^6 steps, and FP32 rounding alone changes the count 2-3 fold).
-    Then an open loop: nreq requests arriving as a Poisson stream at load x capacity,
+    lasts as long as the slowest request and says more about that one request than
+    about the server.  Then an open loop: nreq requests arriving as a Poisson stream at load x capacity,
     in requests per second of the batch's mean work per request; latency = completion -
     arrival (wall clock)."""
     import numpy as np
