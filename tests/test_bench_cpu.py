@@ -49,3 +49,16 @@ def test_c5_requests_relief_scales_with_extent():
         assert 0.25 * cap <= relief <= cap * (1 + 1e-12), (c.nx, relief, cap)
         seen.add(c.nx)
     assert seen == {128, 256, 512}
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """The build runs on one GPU: --gpus above 1 exits with status 2 and says why,
+    before it touches a device."""
+    import subprocess
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
