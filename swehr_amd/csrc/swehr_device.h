@@ -764,6 +764,11 @@ __device__ __forceinline__ void wave_speed_tile(const StepArgs<R> &A, int tile)
 template <typename R, bool SPLIT = false>
 constexpr int kStepWaves = sizeof(R) == 4 ? (SPLIT ? 4 : 5) : 3;
 
+// Size classes whose halo the step kernel loads per batch in phase 1 (FP32 keeps fewer
+// in flight to stay at 5 waves per SIMD).
+template <typename R>
+constexpr int kHaloBatch = sizeof(R) == 4 ? 4 : 8;
+
 // FP32: classes whose M_k the step kernel loads with the own-cell values (the rest
 // one class ahead in the class loop); the 96-VGPR budget of 5 waves/SIMD spills from
 // 2 on.
@@ -835,7 +840,10 @@ struct TileGeom {
 // index is then passed through an empty asm statement, so that the compiler cannot
 // hoist the per-thread index arithmetic out of the loop, where it would stay live
 // through every tile's step and crowd its registers.
-template <typename R, int TI, int TJ, bool SPLIT, int KOWN, bool LOOP = false>
+// PF: more classes than one batch of halo loads (kHaloBatch): the next batch's first
+// classes are loaded before the current batch is converted (a variant of its own, so
+// that the kernel for fewer classes keeps its code).
+template <typename R, int TI, int TJ, bool SPLIT, int KOWN, bool LOOP = false, bool PF = false>
 __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsigned char *smem_raw)
 {
     using T = TileGeom<R, TI, TJ>;
@@ -962,7 +970,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
         }
     }
     // suspended mass of the ext cells, up to KC classes per batch of loads
-    constexpr int KC = sizeof(R) == 4 ? 4 : 8;  // FP32 keeps fewer in flight to stay at 5 waves per SIMD
+    constexpr int KC = kHaloBatch<R>;
     R ch_r[KC][EIT];
 #pragma unroll
     for (int kk = 0; kk < KC; kk++)
@@ -1012,13 +1020,28 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             s_dep[e] = dep_r[r];
         }
     }
+    // PF: the next batch's first KP classes are loaded before this batch is converted,
+    // so that their latency overlaps the conversion (the registers of the depth and
+    // momenta above are free by then)
+    constexpr int KP = PF ? 2 : 0;
+    R ch_p[KP > 0 ? KP : 1][EIT];
     for (int k0 = 0; k0 < S; k0 += KC) {
         if (k0 > 0) {
 #pragma unroll
             for (int kk = 0; kk < KC; kk++)
 #pragma unroll
                 for (int r = 0; r < EIT; r++)
-                    if (k0 + kk < S && tid + r * NT < EXT) ch_r[kk][r] = at(chin + (k0 + kk) * plane, c_r[r]);
+                    if (k0 + kk < S && tid + r * NT < EXT) {
+                        if (kk < KP) ch_r[kk][r] = ch_p[kk < KP ? kk : 0][r];
+                        else ch_r[kk][r] = at(chin + (k0 + kk) * plane, c_r[r]);
+                    }
+        }
+        if (KP > 0 && k0 + KC < S) {
+#pragma unroll
+            for (int kk = 0; kk < KP; kk++)
+#pragma unroll
+                for (int r = 0; r < EIT; r++)
+                    if (k0 + KC + kk < S && tid + r * NT < EXT) ch_p[kk][r] = at(chin + (k0 + KC + kk) * plane, c_r[r]);
         }
 #pragma unroll
         for (int kk = 0; kk < KC; kk++) {
@@ -1306,7 +1329,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     }
 }
 
-template <typename R, int TI, int TJ, bool SPLIT>
+template <typename R, int TI, int TJ, bool SPLIT, bool PF = false>
 __global__ __launch_bounds__(TI *TJ, (kStepWaves<R, SPLIT>)) void step_kernel(StepArgs<R> A)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -1315,7 +1338,7 @@ __global__ __launch_bounds__(TI *TJ, (kStepWaves<R, SPLIT>)) void step_kernel(St
     const int b = blockIdx.x;
     const int tile = (b & 7) * A.blocks_per_xcd + (b >> 3);
     if (tile >= A.nblocks) return;
-    step_tile<R, TI, TJ, SPLIT, fp64_own<TI, TJ, false>()>(A, tile, smem_raw);
+    step_tile<R, TI, TJ, SPLIT, fp64_own<TI, TJ, false>(), false, PF>(A, tile, smem_raw);
 }
 
 // --------------------------------------------------------------------------- K3
