@@ -878,8 +878,9 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
 
     double *s_z = reinterpret_cast<double *>(smem_raw), *s_zp = s_z + ZEXT;
     R *s_h = reinterpret_cast<R *>(s_zp + ZEXT);
-    __shared__ double s_exp2[32];  // 2^(j/32) for the FP64 exponentials of phase 3
-    __shared__ double s_log[69];   // kLogTab for the FP64 logarithms of phase 3
+    // 2^(j/32) for the FP64 exponentials of phase 3, then kLogTab for its logarithms
+    __shared__ double s_tab[32 + 69];
+    double *const s_exp2 = s_tab, *const s_log = s_tab + 32;
     R *s_u = s_h + EXT, *s_v = s_u + EXT, *s_ts = s_v + EXT, *s_dep = s_ts + EXT;
     R *s_fxw = s_dep + EXT, *s_fye = s_fxw + 3 * NFXW;
     R *s_sed = s_fye + 3 * NFYE;
@@ -888,8 +889,27 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
     const R inv_rhos = R(1.0 / kRhos);
     int tid = threadIdx.x;
     if constexpr (LOOP) asm volatile("" : "+v"(tid));
-    if (sizeof(R) == sizeof(double) && tid < 32) s_exp2[tid] = kExp2Frac[tid];
-    if (sizeof(R) == sizeof(double) && tid < 69) s_log[tid] = (&kLogTab[0][0])[tid];
+    // The tables go to LDS by direct (LDS-DMA) loads of the first wave, one dword per
+    // lane (lane L to byte 4L after the instruction's base): no registers and no wait
+    // until the barrier of phase 1 (a copy through registers waits on each load before
+    // the wave issues any of the tile's loads).
+    // (The prefetching variant copies them through registers: with the direct loads its
+    // register allocation spills.)
+    if constexpr (sizeof(R) == sizeof(double) && PF) {
+        if (tid < 32) s_exp2[tid] = kExp2Frac[tid];
+        if (tid < 69) s_log[tid] = (&kLogTab[0][0])[tid];
+    } else if constexpr (sizeof(R) == sizeof(double)) {
+        if (tid < 64) {
+            typedef __attribute__((address_space(3))) void lds_void;
+            const unsigned *e2 = reinterpret_cast<const unsigned *>(kExp2Frac);
+            const unsigned *lg = reinterpret_cast<const unsigned *>(&kLogTab[0][0]);
+            __builtin_amdgcn_global_load_lds(e2 + tid, (lds_void *)s_tab, 4, 0, 0);  // 32 doubles
+#pragma unroll
+            for (int c = 0; c < 3; c++)  // 69 doubles = 138 dwords
+                if (c * 64 + tid < 138)
+                    __builtin_amdgcn_global_load_lds(lg + c * 64 + tid, (lds_void *)((char *)s_log + c * 256), 4, 0, 0);
+        }
+    }
 
     // Own-cell values of the update, loaded first so that their latency overlaps the
     // halo staging below (every load of a phase is issued before the first wait).
@@ -1074,6 +1094,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &A, int tile, unsign
             s_zp[e] = zp_r[r];
         }
     }
+    if constexpr (sizeof(R) == sizeof(double) && !PF) __builtin_amdgcn_s_waitcnt(0);  // the tables' LDS-DMA loads
     __syncthreads();
 
     const double odxd = A.inv_dx;
