@@ -67,13 +67,13 @@ size_t tile_lds(int id, int S)
     }
 }
 
-template <typename R, bool SPLIT, bool PF = false>
+template <typename R, bool SPLIT>
 void launch_step_v(int id, int blocks, size_t lds, hipStream_t s, const StepArgs<R> &A)
 {
     switch (id) {
-    case 1: hipLaunchKernelGGL((step_kernel<R, 8, 32, SPLIT, PF>), dim3(blocks), dim3(256), lds, s, A); break;
-    case 2: hipLaunchKernelGGL((step_kernel<R, 4, 64, SPLIT, PF>), dim3(blocks), dim3(256), lds, s, A); break;
-    case 3: hipLaunchKernelGGL((step_kernel<R, 16, 16, SPLIT, PF>), dim3(blocks), dim3(256), lds, s, A); break;
+    case 1: hipLaunchKernelGGL((step_kernel<R, 8, 32, SPLIT>), dim3(blocks), dim3(256), lds, s, A); break;
+    case 2: hipLaunchKernelGGL((step_kernel<R, 4, 64, SPLIT>), dim3(blocks), dim3(256), lds, s, A); break;
+    case 3: hipLaunchKernelGGL((step_kernel<R, 16, 16, SPLIT>), dim3(blocks), dim3(256), lds, s, A); break;
     default: hipLaunchKernelGGL((step_kernel<R, 4, 32, SPLIT>), dim3(blocks), dim3(128), lds, s, A); break;
     }
 }
@@ -91,7 +91,8 @@ bool use_split(int id, int S, size_t lds)
 
 // The prefetching variant (PF) where FP64 classes span two halo batches in one LDS chunk
 // (4096^2, S = 10: step kernel 2.028 -> 1.995 ms; FP32 and fewer classes gain nothing).
-// The 128-thread tile keeps the plain variant (PF spills there).
+// The 128-thread tile keeps the plain variant (PF spills there).  Its launcher is in
+// swehr_step_pf.hip, built with a scheduling strategy of its own.
 template <typename R>
 bool use_prefetch(int id, int S, bool split)
 {
@@ -108,7 +109,7 @@ void launch_step(int id, int blocks, size_t lds, hipStream_t s, const StepArgs<R
     }
     if constexpr (sizeof(R) == sizeof(double)) {
         if (use_prefetch<R>(id, A.S, split)) {
-            launch_step_v<R, false, true>(id, blocks, lds, s, A);
+            launch_step_prefetch(id, blocks, lds, s, A);
             return;
         }
     }

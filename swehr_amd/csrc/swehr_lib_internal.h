@@ -1,5 +1,6 @@
 // swehr_lib_internal.h - definitions shared by the host-side translation units
-// (swehr_lib.hip: single instance; swehr_batch.hip: ensembles and serving).
+// (swehr_lib.hip: single instance; swehr_step_pf.hip: the prefetching FP64 step
+// kernel's launcher; swehr_batch.hip: ensembles and serving).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -10,6 +11,11 @@
 
 // Record the calling thread's last error (returned by swehr_last_error()).
 void swehr_set_error(const char *fmt, const char *a, const char *b);
+
+namespace swehr {
+// Launch the prefetching FP64 step kernel on tile id 1-3 (swehr_step_pf.hip).
+void launch_step_prefetch(int id, int blocks, size_t lds, hipStream_t s, const StepArgs<double> &A);
+}  // namespace swehr
 
 #define SWEHR_HIPCHK(x)                                                                        \
     do {                                                                                       \
